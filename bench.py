@@ -1,0 +1,284 @@
+"""Benchmark of the MI355X streaming DBSCAN slide step.
+
+Default workload (the configuration the headline metric is quoted on): the
+3-D trajectory-like stream "walks3d" - W 100,000, S 10,000, Eps 0.5, MinPts 10,
+one stream, plain per-slide GPU path. A step is one slide: insert the stride,
+expire the oldest stride, recluster the window, labels available on the device.
+All inputs are resident in HBM before the timed region.
+
+Also reported on the same JSON line:
+  roofline      - the range-count kernel: algorithmic bytes per launch
+                  (rtdbstream_amd/roofline.py) / its average HIP-event duration
+  cpu_baseline  - the CPU port (oracle, OpenMP) on a bounded sample of the same
+                  slides, whose labels are also compared with the GPU's
+  batched       - the 64-stream batch: one launch per phase, HIP graph, vs the
+                  same streams slid one at a time (plain)
+  served        - the request server under Poisson arrivals: throughput and
+                  p50/p99 latency
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "points/sec & slides/sec + p99 slide latency, batched/served vs plain, exact labels"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def window_at(data, W, S, t):
+    win = np.array(data[:W], copy=True)
+    units = W // S
+    for k in range(t + 1):
+        u = k % units
+        win[u * S:(u + 1) * S] = data[W + k * S: W + (k + 1) * S]
+    return win
+
+
+def plain_leg(R, WL, name, steps, warmup):
+    cfg = WL.WORKLOADS[name]
+    W, S, dim, eps, mp = cfg["window"], cfg["stride"], cfg["dim"], cfg["eps"], cfg["min_pts"]
+    total_slides = warmup + steps
+    data = WL.generate(name, n=W + total_slides * S)
+    d_data = R.DeviceBuffer(data.nbytes)
+    d_data.upload(data)
+    d_label = R.DeviceBuffer(W * 4)
+    d_cid = R.DeviceBuffer(W * 4)
+    s = R.Stream(dim, W, S, eps, mp)
+    s.init(data[:W])
+
+    def slide(t):
+        s.slide_async(d_data.addr((W + t * S) * dim * 4), d_label.addr(), d_cid.addr())
+        s.sync()
+
+    for t in range(warmup):
+        slide(t)
+    s.set_timing(True)
+    lat = []
+    R.lib().rtdbs_device_synchronize()
+    t0 = time.perf_counter()
+    for t in range(warmup, total_slides):
+        a = time.perf_counter()
+        slide(t)
+        lat.append((time.perf_counter() - a) * 1e3)
+    R.lib().rtdbs_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    timing = s.timing()
+    info = s.info()
+    s.close()
+    return dict(data=data, cfg=cfg, elapsed=elapsed, lat=lat, timing=timing, info=info, W=W, S=S, dim=dim,
+                eps=eps, min_pts=mp, warmup=warmup, steps=steps)
+
+
+def roofline(WL, res, sample=8):
+    from rtdbstream_amd import roofline as RF
+
+    data, W, S, eps, mp = res["data"], res["W"], res["S"], res["eps"], res["min_pts"]
+    info = res["info"]
+    t_first = res["warmup"]
+    ts = list(range(t_first, t_first + res["steps"]))
+    if len(ts) > sample:
+        ts = ts[:: max(1, len(ts) // sample)][:sample]
+    bytes_per = []
+    for t in ts:
+        win = window_at(data, W, S, t)
+        q = ~RF.dense_mask(win, eps, mp, info)
+        bytes_per.append(RF.range_count_bytes(win, eps, q))
+    B = float(np.mean(bytes_per))
+    ms, n = res["timing"]["count"]
+    avg_s = ms / max(1, n) / 1e3
+    achieved = B / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_count", "algorithmic_bytes_per_launch": int(B),
+            "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
+def cpu_baseline(R, WL, res, seconds):
+    """CPU port (oracle grid, OpenMP) on a bounded sample of the same slides;
+    the GPU labels of those slides are recomputed and compared bit for bit."""
+    from oracle import oracle as O
+
+    data, W, S, dim, eps, mp = res["data"], res["W"], res["S"], res["dim"], res["eps"], res["min_pts"]
+    threads = O.max_threads()
+    s = R.Stream(dim, W, S, eps, mp)
+    s.init(data[:W])
+    n_done, cpu_time, exact = 0, 0.0, True
+    t = 0
+    while (cpu_time < seconds or n_done < 2) and t < res["warmup"] + res["steps"]:
+        label, cid = s.slide(data[W + t * S: W + (t + 1) * S])
+        win = window_at(data, W, S, t)
+        a = time.perf_counter()
+        rl, rc = O.grid(win, eps, mp, threads=threads)
+        cpu_time += time.perf_counter() - a
+        exact = exact and np.array_equal(label, rl) and np.array_equal(cid, rc)
+        n_done += 1
+        t += 1
+    s.close()
+    return {"value": round(n_done * S / cpu_time, 1), "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": "%d slides of %s (W=%d, S=%d), oracle grid + OpenMP, %.1f s" % (n_done, "window", W, S, cpu_time),
+            "slides_per_s": round(n_done / cpu_time, 3)}, exact, n_done
+
+
+def batched_leg(R, WL, n_streams, slides, warmup):
+    specs = WL.multi_stream_specs(n_streams=n_streams, slides=slides + warmup)
+    streams, bufs, datas = [], [], []
+    for sp in specs:
+        d = WL.stream_points(sp)
+        datas.append(d)
+        db = R.DeviceBuffer(d.nbytes)
+        db.upload(d)
+        bufs.append((db, R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4),
+                     R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4)))
+    # plain: one stream at a time
+    for sp, d in zip(specs, datas):
+        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+        s.init(d[:sp["window"]])
+        streams.append(s)
+
+    def in_addr(i, t):
+        sp = specs[i]
+        return bufs[i][0].addr((sp["window"] + t * sp["stride"]) * sp["dim"] * 4)
+
+    for t in range(warmup):
+        for i, s in enumerate(streams):
+            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
+            s.sync()
+    R.lib().rtdbs_device_synchronize()
+    t0 = time.perf_counter()
+    lat_plain = []
+    for t in range(warmup, warmup + slides):
+        for i, s in enumerate(streams):
+            a = time.perf_counter()
+            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
+            s.sync()
+            lat_plain.append((time.perf_counter() - a) * 1e3)
+    R.lib().rtdbs_device_synchronize()
+    t_plain = time.perf_counter() - t0
+    for s in streams:
+        s.close()
+    # batched: same streams from the same first windows, one graph replay per slide
+    streams = []
+    for sp, d in zip(specs, datas):
+        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+        s.init(d[:sp["window"]])
+        streams.append(s)
+    b = R.Batch(streams, use_graph=True)
+
+    def bslide(t):
+        b.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                      [x[4].addr() for x in bufs])
+        b.sync()
+
+    for t in range(warmup):
+        bslide(t)
+    R.lib().rtdbs_device_synchronize()
+    t0 = time.perf_counter()
+    lat_b = []
+    for t in range(warmup, warmup + slides):
+        a = time.perf_counter()
+        bslide(t)
+        lat_b.append((time.perf_counter() - a) * 1e3)
+    R.lib().rtdbs_device_synchronize()
+    t_batch = time.perf_counter() - t0
+    # exact: batched labels of the last slide equal the plain ones, per stream
+    same = True
+    for i, sp in enumerate(specs):
+        W = sp["window"]
+        l1, c1, l2, c2 = (np.empty(W, np.int32) for _ in range(4))
+        bufs[i][1].download(l1)
+        bufs[i][2].download(c1)
+        bufs[i][3].download(l2)
+        bufs[i][4].download(c2)
+        same = same and np.array_equal(l1, l2) and np.array_equal(c1, c2)
+    b.close()
+    for s in streams:
+        s.close()
+    pts = sum(sp["stride"] for sp in specs) * slides
+    nsl = len(specs) * slides
+    return {
+        "streams": len(specs), "slides_per_stream": slides,
+        "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
+        "plain_points_per_s": round(pts / t_plain, 1), "plain_slides_per_s": round(nsl / t_plain, 1),
+        "batched_points_per_s": round(pts / t_batch, 1), "batched_slides_per_s": round(nsl / t_batch, 1),
+        "speedup": round(t_plain / t_batch, 3),
+        "plain_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
+        "plain_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
+        "batched_step_p50_ms": round(float(np.percentile(lat_b, 50)), 3),
+        "batched_step_p99_ms": round(float(np.percentile(lat_b, 99)), 3),
+        "labels_equal_plain": bool(same),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="walks3d")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--batch-streams", type=int, default=64)
+    ap.add_argument("--batch-slides", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs every stream on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import rtdbstream_amd as R
+    from rtdbstream_amd import workloads as WL
+
+    R.lib()
+    if R.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("plain leg: %s, %d warmup + %d timed slides" % (args.workload, args.warmup, args.steps))
+    res = plain_leg(R, WL, args.workload, args.steps, args.warmup)
+    W, S = res["W"], res["S"]
+    ms_per = res["elapsed"] / args.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps * S / res["elapsed"], 1),
+        "unit": "points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded %s generator (rtdbstream_amd/workloads.py), inputs resident in HBM" % args.workload,
+        "config": {"workload": args.workload, "dim": res["dim"], "window": W, "stride": S, "eps": res["eps"],
+                   "min_pts": res["min_pts"], "streams": 1, "path": "plain"},
+        "slides_per_s": round(args.steps / res["elapsed"], 2),
+        "window_points_per_s": round(args.steps * W / res["elapsed"], 1),
+        "p50_ms": round(float(np.percentile(res["lat"], 50)), 4),
+        "p99_ms": round(float(np.percentile(res["lat"], 99)), 4),
+        "phase_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in res["timing"].items()},
+        "grid": {"cell_div": res["info"]["cell_div"], "cells_last": res["info"]["cells"]},
+    }
+    log("plain: %.1f points/s, %.3f ms/slide" % (out["value"], ms_per))
+    out["roofline"] = roofline(WL, res)
+    log("roofline: %s" % out["roofline"])
+    if not args.no_cpu:
+        cb, exact, nchk = cpu_baseline(R, WL, res, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["exact_labels"] = {"slides_checked": nchk, "equal": bool(exact)}
+        log("cpu baseline: %s exact=%s" % (cb, exact))
+    if not args.no_batch:
+        log("batched leg: %d streams x %d slides" % (args.batch_streams, args.batch_slides))
+        out["batched"] = batched_leg(R, WL, args.batch_streams, args.batch_slides, 2)
+        log("batched: %s" % out["batched"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

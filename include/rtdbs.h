@@ -1,0 +1,217 @@
+/*
+ * rtdbs.h - C ABI of the MI355X streaming DBSCAN slide engine.
+ *
+ * Drop-in for the per-slide step of RTDBStream's sliding-window DBSCAN
+ * (reference: src/dbscan/dbscan.cpp:2258-2364, the OPTIMIZATION_LEVEL 9
+ * pipeline). The window-slide API keeps the reference's parameters
+ * (window W, stride S, Eps, MinPts) and its per-slide output: two slot-indexed
+ * int32 arrays of length W,
+ *   label[slot] in {0 core, 1 border, 2 noise}   (reference dbscan.h:27)
+ *   cid[slot]   = cluster id: for cores the minimum core slot of the cluster,
+ *                 for borders the minimum such id among clusters with a core
+ *                 within Eps, for noise the point's own slot.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Points are FP32, point-major, `dim`
+ *    floats per point (dim 1, 2 or 3; the reference pads to 3 with zeros,
+ *    util.cpp:146,194,238-239, which gives identical distances).
+ *  - Return codes: 0 = OK, negative = error (RTDBS_E*); nothing is thrown
+ *    across the ABI. rtdbs_last_error() returns a thread-local message.
+ *  - The caller owns host buffers; the engine owns all device state.
+ *  - A stream or batch handle is used by one host thread at a time. The
+ *    server object is thread-safe.
+ *  - "_async" entry points take DEVICE pointers and enqueue work on the
+ *    handle's own HIP stream; call the matching _sync before reading results
+ *    or reusing the inputs.
+ */
+#ifndef RTDBS_H
+#define RTDBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTDBS_OK 0
+#define RTDBS_EINVAL -1     /* invalid argument (S does not divide W, eps <= 0, ...) */
+#define RTDBS_ENOMEM -2     /* device or host allocation failed */
+#define RTDBS_EHIP -3       /* HIP runtime error */
+#define RTDBS_ESTATE -4     /* call not valid in the handle's state (e.g. slide before init) */
+#define RTDBS_EBUDGET -5    /* server: admission refused by the HBM budget */
+#define RTDBS_ETIMEOUT -6   /* server: wait timed out */
+#define RTDBS_ECANCELLED -7 /* server: request cancelled */
+#define RTDBS_ENOTFOUND -8  /* server: unknown stream or ticket */
+
+#define RTDBS_LABEL_CORE 0
+#define RTDBS_LABEL_BORDER 1
+#define RTDBS_LABEL_NOISE 2
+
+/* Stream parameters.
+ * Replaces the reference's ScanState/Params set-up (state.h:23-129,
+ * dbscan.h:15-90, initialize_params at dbscan.cpp:30-132). */
+typedef struct rtdbs_params {
+    int32_t dim;      /* 1, 2 or 3 coordinates per point */
+    int32_t window;   /* W: points in the window (reference -W, util.cpp:463) */
+    int32_t stride;   /* S: points per slide, must divide W (ring: dbscan.cpp:2260,2289) */
+    float eps;        /* Eps (reference -R, narrowed to float: util.cpp:491, state.h:68) */
+    int32_t min_pts;  /* MinPts, self included (reference -K, util.cpp:480; aabb.cu:153) */
+    float lo[3];      /* grid bounds (reference: whole-file min/max, util.cpp:23-55);   */
+    float hi[3];      /* if hi[a] <= lo[a] on a used axis they come from the first window */
+    int32_t cell_div; /* stencil reach k (grid side = Eps/k), 1..3; 0 = autotuned default */
+    int32_t reserved[7];
+} rtdbs_params;
+
+typedef struct rtdbs_stream rtdbs_stream;
+typedef struct rtdbs_batch rtdbs_batch;
+typedef struct rtdbs_server rtdbs_server;
+
+typedef struct rtdbs_stream_info {
+    int64_t slides;        /* slides completed */
+    int32_t cells;         /* non-empty grid cells in the current window */
+    int32_t cell_div;      /* stencil reach k in use */
+    int32_t rows;          /* stencil rows per cell */
+    int32_t dense_shortcut;/* 1 if cells are small enough that >= MinPts points => all core */
+    int64_t grid[3];       /* grid cells per axis */
+    double cell_side;      /* grid side length */
+    double lo[3];          /* grid origin */
+    size_t device_bytes;   /* device memory held by the stream */
+} rtdbs_stream_info;
+
+/* ---- single stream ("plain" path) --------------------------------------- */
+
+/* Validates parameters and allocates the stream's device state.
+ * Replaces initialize_params (dbscan.cpp:30-132). */
+int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **out);
+
+/* Device bytes a stream with these parameters holds (for HBM budgeting). */
+int rtdbs_stream_device_bytes(const rtdbs_params *params, size_t *bytes);
+
+/* Loads the first window (W points, host memory). No labels are produced,
+ * as in the reference (dbscan.cpp:2268-2272: the first window is loaded and
+ * gridded but not clustered). Resets the ring cursor. */
+int rtdbs_stream_init(rtdbs_stream *s, const float *points);
+
+/* Restores a stream from a checkpoint: the window in slot order (W*dim
+ * floats, as rtdbs_stream_get_window returns it) and the number of slides
+ * already done, which fixes the ring cursor u = slides mod (W/S). The
+ * reference keeps no checkpoint; a window is fully determined by its W
+ * points and the cursor, so the labels that follow are identical. */
+int rtdbs_stream_restore(rtdbs_stream *s, const float *window_points, int64_t slides_done);
+
+/* One slide, synchronous, host buffers: inserts S points (host) into the
+ * ring slots [u*S, u*S+S), expires the S oldest, clusters the window and
+ * writes label[W], cid[W] (host). Replaces the loop body
+ * dbscan.cpp:2276-2362 (H2D of the stride at :2289 through D2H at
+ * :2344-2345). */
+int rtdbs_stream_slide(rtdbs_stream *s, const float *stride_points, int32_t *label, int32_t *cid);
+
+/* Same slide with device buffers, enqueued on the stream's HIP stream. */
+int rtdbs_stream_slide_async(rtdbs_stream *s, const float *stride_points_dev, int32_t *label_dev,
+                             int32_t *cid_dev);
+int rtdbs_stream_sync(rtdbs_stream *s);
+int rtdbs_stream_get_info(const rtdbs_stream *s, rtdbs_stream_info *info);
+/* Copies the current window back to the host in slot order (W*dim floats). */
+int rtdbs_stream_get_window(rtdbs_stream *s, float *points);
+void rtdbs_stream_destroy(rtdbs_stream *s);
+
+/* ---- batched streams (one launch per phase, optional HIP graph) ---------- */
+/* No reference counterpart: the reference slides one stream at a time. */
+
+/* Groups n initialised streams. With use_graph != 0 the slide step is
+ * captured once into a HIP graph and replayed. A stream may belong to
+ * several batches but must not be slid by two of them concurrently. */
+int rtdbs_batch_create(rtdbs_stream *const *streams, int n, int use_graph, rtdbs_batch **out);
+/* One slide of every stream in the batch; host buffers, synchronous. */
+int rtdbs_batch_slide(rtdbs_batch *b, const float *const *stride_points, int32_t *const *labels,
+                      int32_t *const *cids);
+/* Device buffers, enqueued on the batch's HIP stream. */
+int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *stride_points_dev,
+                            int32_t *const *labels_dev, int32_t *const *cids_dev);
+int rtdbs_batch_sync(rtdbs_batch *b);
+void rtdbs_batch_destroy(rtdbs_batch *b);
+
+/* ---- request server (continuous batching, HBM-budget admission) ---------- */
+/* No reference counterpart. */
+
+typedef struct rtdbs_server_config {
+    size_t hbm_budget_bytes; /* admission limit on the sum of stream device bytes */
+    int32_t max_batch;       /* most streams slid together in one launch */
+    int32_t lanes;           /* batches that may be on the device at once (HIP streams) */
+    int32_t use_graph;       /* capture recurring batch shapes as HIP graphs */
+    int32_t admit_wait_ms;   /* open_stream waits this long for budget to free up */
+    int32_t reserved[4];
+} rtdbs_server_config;
+
+typedef struct rtdbs_server_stats {
+    int64_t completed;
+    int64_t failed;
+    int64_t batches;
+    double mean_batch;
+    double p50_ms;
+    double p99_ms;
+    double max_ms;
+    size_t bytes_in_use;
+    int32_t streams;
+    int32_t reserved;
+} rtdbs_server_stats;
+
+int rtdbs_server_create(const rtdbs_server_config *cfg, rtdbs_server **out);
+/* Admits a stream (and loads its first window, host memory) if its device
+ * bytes fit the budget, else RTDBS_EBUDGET. */
+int rtdbs_server_open_stream(rtdbs_server *srv, const rtdbs_params *params, const float *first_window,
+                             int32_t *stream_id);
+/* Releases the stream's device memory. Pending requests are cancelled. */
+int rtdbs_server_close_stream(rtdbs_server *srv, int32_t stream_id);
+/* Checkpoints a stream to host memory and frees its device state; its
+ * requests wait. rtdbs_server_readmit rebuilds it (subject to the budget). */
+int rtdbs_server_evict_stream(rtdbs_server *srv, int32_t stream_id);
+int rtdbs_server_readmit_stream(rtdbs_server *srv, int32_t stream_id);
+/* Queues one slide of a stream. stride_points (S*dim floats), label and cid
+ * (W ints each) are host buffers that must stay valid until the ticket
+ * completes. Slides of one stream run in submission order. */
+int rtdbs_server_submit(rtdbs_server *srv, int32_t stream_id, const float *stride_points,
+                        int32_t *label, int32_t *cid, int64_t *ticket);
+/* Waits for a ticket (timeout_ms < 0: forever). Returns the request's status. */
+int rtdbs_server_wait(rtdbs_server *srv, int64_t ticket, int32_t timeout_ms);
+/* Non-blocking: 1 done, 0 pending, negative error. */
+int rtdbs_server_poll(rtdbs_server *srv, int64_t ticket);
+/* Cancels a queued request (RTDBS_ESTATE once it is on the device). */
+int rtdbs_server_cancel(rtdbs_server *srv, int64_t ticket);
+int rtdbs_server_get_stats(rtdbs_server *srv, rtdbs_server_stats *stats);
+int rtdbs_server_reset_stats(rtdbs_server *srv);
+void rtdbs_server_destroy(rtdbs_server *srv);
+
+/* ---- device memory helpers (so callers need no other GPU runtime) -------- */
+int rtdbs_device_alloc(size_t bytes, void **ptr);
+int rtdbs_device_free(void *ptr);
+int rtdbs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
+int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+int rtdbs_device_synchronize(void);
+int rtdbs_device_count(int *count);
+
+/* ---- kernel timing (HIP events on the engine's own streams) -------------- */
+/* Enables per-phase event timing on a batch (or a stream's solo batch). */
+int rtdbs_batch_set_timing(rtdbs_batch *b, int enable);
+int rtdbs_stream_set_timing(rtdbs_stream *s, int enable);
+/* Accumulated milliseconds and launch counts per phase since the last reset.
+ * names: comma-separated phase names written to a caller buffer. */
+int rtdbs_batch_get_timing(rtdbs_batch *b, double *ms, int64_t *launches, int max_phases,
+                           char *names, size_t names_len);
+int rtdbs_stream_get_timing(rtdbs_stream *s, double *ms, int64_t *launches, int max_phases,
+                            char *names, size_t names_len);
+
+/* Test hook: the next `count` stream creations fail with RTDBS_ENOMEM before
+ * touching the device (host-forced allocation failure). */
+int rtdbs_test_fail_allocations(int count);
+
+/* Thread-local description of the last error on this thread. */
+const char *rtdbs_last_error(void);
+const char *rtdbs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTDBS_H */

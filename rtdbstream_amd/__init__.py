@@ -1,0 +1,430 @@
+"""MI355X streaming DBSCAN slide engine - Python bindings over the C ABI.
+
+The engine (librtdbs.so: C++ host code + HIP kernels for gfx950) implements the
+per-slide step of RTDBStream's sliding-window DBSCAN (reference
+src/dbscan/dbscan.cpp:2258-2364): each slide inserts S points into a ring of W
+slots, expires the S oldest, and returns slot-indexed `label` (0 core,
+1 border, 2 noise) and `cid` arrays.
+
+These bindings load the in-tree library and fail loudly if it is missing: there
+is no CPU fallback on the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "RtdbsError", "Params", "Stream", "Batch", "Server", "DeviceBuffer", "lib", "library_path",
+    "LABEL_CORE", "LABEL_BORDER", "LABEL_NOISE", "device_count",
+]
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_PKG, "librtdbs.so")
+
+LABEL_CORE, LABEL_BORDER, LABEL_NOISE = 0, 1, 2
+
+OK = 0
+EINVAL, ENOMEM, EHIP, ESTATE, EBUDGET, ETIMEOUT, ECANCELLED, ENOTFOUND = -1, -2, -3, -4, -5, -6, -7, -8
+_NAMES = {EINVAL: "EINVAL", ENOMEM: "ENOMEM", EHIP: "EHIP", ESTATE: "ESTATE", EBUDGET: "EBUDGET",
+          ETIMEOUT: "ETIMEOUT", ECANCELLED: "ECANCELLED", ENOTFOUND: "ENOTFOUND"}
+
+
+class RtdbsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_NAMES.get(code, "E?"), code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("window", ctypes.c_int32), ("stride", ctypes.c_int32),
+        ("eps", ctypes.c_float), ("min_pts", ctypes.c_int32),
+        ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+        ("cell_div", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class StreamInfo(ctypes.Structure):
+    _fields_ = [
+        ("slides", ctypes.c_int64), ("cells", ctypes.c_int32), ("cell_div", ctypes.c_int32),
+        ("rows", ctypes.c_int32), ("dense_shortcut", ctypes.c_int32), ("grid", ctypes.c_int64 * 3),
+        ("cell_side", ctypes.c_double), ("lo", ctypes.c_double * 3), ("device_bytes", ctypes.c_size_t),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int32), ("lanes", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32), ("admit_wait_ms", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_int64), ("failed", ctypes.c_int64), ("batches", ctypes.c_int64),
+        ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double), ("bytes_in_use", ctypes.c_size_t), ("streams", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F32P = ctypes.POINTER(ctypes.c_float)
+
+# (name, restype, argtypes) of every exported entry point of include/rtdbs.h
+SIGNATURES = [
+    ("rtdbs_stream_create", ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_P)]),
+    ("rtdbs_stream_device_bytes", ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_size_t)]),
+    ("rtdbs_stream_init", ctypes.c_int, [_P, _P]),
+    ("rtdbs_stream_restore", ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    ("rtdbs_stream_slide", ctypes.c_int, [_P, _P, _P, _P]),
+    ("rtdbs_stream_slide_async", ctypes.c_int, [_P, _P, _P, _P]),
+    ("rtdbs_stream_sync", ctypes.c_int, [_P]),
+    ("rtdbs_stream_get_info", ctypes.c_int, [_P, ctypes.POINTER(StreamInfo)]),
+    ("rtdbs_stream_get_window", ctypes.c_int, [_P, _P]),
+    ("rtdbs_stream_destroy", None, [_P]),
+    ("rtdbs_batch_create", ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    ("rtdbs_batch_slide", ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    ("rtdbs_batch_slide_async", ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    ("rtdbs_batch_sync", ctypes.c_int, [_P]),
+    ("rtdbs_batch_destroy", None, [_P]),
+    ("rtdbs_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    ("rtdbs_server_open_stream", ctypes.c_int, [_P, ctypes.POINTER(Params), _P, _I32P]),
+    ("rtdbs_server_close_stream", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("rtdbs_server_evict_stream", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("rtdbs_server_readmit_stream", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("rtdbs_server_submit", ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
+    ("rtdbs_server_wait", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
+    ("rtdbs_server_poll", ctypes.c_int, [_P, ctypes.c_int64]),
+    ("rtdbs_server_cancel", ctypes.c_int, [_P, ctypes.c_int64]),
+    ("rtdbs_server_get_stats", ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    ("rtdbs_server_reset_stats", ctypes.c_int, [_P]),
+    ("rtdbs_server_destroy", None, [_P]),
+    ("rtdbs_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    ("rtdbs_device_free", ctypes.c_int, [_P]),
+    ("rtdbs_memcpy_h2d", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("rtdbs_memcpy_d2h", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("rtdbs_device_synchronize", ctypes.c_int, []),
+    ("rtdbs_device_count", ctypes.c_int, [_I32P]),
+    ("rtdbs_batch_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
+    ("rtdbs_stream_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
+    ("rtdbs_batch_get_timing", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+    ("rtdbs_stream_get_timing", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+    ("rtdbs_test_fail_allocations", ctypes.c_int, [ctypes.c_int]),
+    ("rtdbs_last_error", ctypes.c_char_p, []),
+    ("rtdbs_version", ctypes.c_char_p, []),
+]
+
+_lib = None
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Loads librtdbs.so (built by __graft_entry__.build()). Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("librtdbs.so is not built (%s); run __graft_entry__.build()" % _LIB_PATH)
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise RtdbsError(rc, lib().rtdbs_last_error().decode(errors="replace"))
+    return rc
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    _check(lib().rtdbs_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0):
+    p = Params()
+    p.dim, p.window, p.stride, p.eps, p.min_pts = int(dim), int(window), int(stride), float(eps), int(min_pts)
+    for a in range(3):
+        p.lo[a] = float(lo[a]) if lo is not None and a < len(lo) else 0.0
+        p.hi[a] = float(hi[a]) if hi is not None and a < len(hi) else 0.0
+    p.cell_div = int(cell_div)
+    return p
+
+
+def _f32(a, n, dim, what):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.size != n * dim:
+        raise ValueError("%s must hold %d points of dim %d (got %d floats)" % (what, n, dim, a.size))
+    return a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by Python (for device-resident inputs)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().rtdbs_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes
+        _check(lib().rtdbs_memcpy_h2d(ctypes.c_void_p(self.ptr + offset), _ptr(arr), arr.nbytes))
+
+    def download(self, out, offset=0):
+        assert offset + out.nbytes <= self.nbytes
+        _check(lib().rtdbs_memcpy_d2h(_ptr(out), ctypes.c_void_p(self.ptr + offset), out.nbytes))
+        return out
+
+    def addr(self, offset=0):
+        return self.ptr + offset
+
+    def free(self):
+        if self.ptr:
+            lib().rtdbs_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """One sliding window (the reference's ScanState + slide loop body)."""
+
+    def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0):
+        self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div)
+        self.dim, self.W, self.S = int(dim), int(window), int(stride)
+        h = ctypes.c_void_p()
+        _check(lib().rtdbs_stream_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h.value
+
+    @staticmethod
+    def device_bytes(dim, window, stride, eps, min_pts, cell_div=0):
+        p = make_params(dim, window, stride, eps, min_pts, cell_div=cell_div)
+        n = ctypes.c_size_t()
+        _check(lib().rtdbs_stream_device_bytes(ctypes.byref(p), ctypes.byref(n)))
+        return n.value
+
+    def init(self, points):
+        pts = _f32(points, self.W, self.dim, "first window")
+        _check(lib().rtdbs_stream_init(self.h, _ptr(pts)))
+
+    def restore(self, window_points, slides_done):
+        pts = _f32(window_points, self.W, self.dim, "window")
+        _check(lib().rtdbs_stream_restore(self.h, _ptr(pts), int(slides_done)))
+
+    def slide(self, stride_points, label=None, cid=None):
+        pts = _f32(stride_points, self.S, self.dim, "stride")
+        if label is None:
+            label = np.empty(self.W, np.int32)
+        if cid is None:
+            cid = np.empty(self.W, np.int32)
+        _check(lib().rtdbs_stream_slide(self.h, _ptr(pts), _ptr(label), _ptr(cid)))
+        return label, cid
+
+    def slide_async(self, stride_dev, label_dev, cid_dev):
+        _check(lib().rtdbs_stream_slide_async(self.h, ctypes.c_void_p(stride_dev), ctypes.c_void_p(label_dev),
+                                              ctypes.c_void_p(cid_dev)))
+
+    def sync(self):
+        _check(lib().rtdbs_stream_sync(self.h))
+
+    def info(self):
+        i = StreamInfo()
+        _check(lib().rtdbs_stream_get_info(self.h, ctypes.byref(i)))
+        return {"slides": i.slides, "cells": i.cells, "cell_div": i.cell_div, "rows": i.rows,
+                "dense_shortcut": i.dense_shortcut, "grid": list(i.grid), "cell_side": i.cell_side,
+                "lo": list(i.lo),
+                "device_bytes": i.device_bytes}
+
+    def window(self):
+        out = np.empty(self.W * self.dim, np.float32)
+        _check(lib().rtdbs_stream_get_window(self.h, _ptr(out)))
+        return out.reshape(self.W, self.dim)
+
+    def set_timing(self, enable=True):
+        _check(lib().rtdbs_stream_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        return _get_timing(lib().rtdbs_stream_get_timing, self.h)
+
+    def close(self):
+        if self.h:
+            lib().rtdbs_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _get_timing(fn, h):
+    ms = (ctypes.c_double * 32)()
+    n = (ctypes.c_int64 * 32)()
+    names = ctypes.create_string_buffer(1024)
+    k = _check(fn(h, ms, n, 32, names, 1024))
+    keys = names.value.decode().split(",")
+    return {keys[i]: (ms[i], n[i]) for i in range(k)}
+
+
+class Batch:
+    """Many streams slid together: one launch per phase, optional HIP graph."""
+
+    def __init__(self, streams, use_graph=True):
+        self.streams = list(streams)
+        arr = (ctypes.c_void_p * len(self.streams))(*[s.h for s in self.streams])
+        h = ctypes.c_void_p()
+        _check(lib().rtdbs_batch_create(arr, len(self.streams), int(use_graph), ctypes.byref(h)))
+        self.h = h.value
+
+    def slide(self, strides, labels=None, cids=None):
+        n = len(self.streams)
+        strides = [_f32(x, s.S, s.dim, "stride") for x, s in zip(strides, self.streams)]
+        if labels is None:
+            labels = [np.empty(s.W, np.int32) for s in self.streams]
+        if cids is None:
+            cids = [np.empty(s.W, np.int32) for s in self.streams]
+        A = (ctypes.c_void_p * n)(*[a.ctypes.data for a in strides])
+        Lb = (ctypes.c_void_p * n)(*[a.ctypes.data for a in labels])
+        C = (ctypes.c_void_p * n)(*[a.ctypes.data for a in cids])
+        _check(lib().rtdbs_batch_slide(self.h, A, Lb, C))
+        return labels, cids
+
+    def slide_async(self, stride_ptrs, label_ptrs, cid_ptrs):
+        n = len(self.streams)
+        A = (ctypes.c_void_p * n)(*stride_ptrs)
+        Lb = (ctypes.c_void_p * n)(*label_ptrs)
+        C = (ctypes.c_void_p * n)(*cid_ptrs)
+        _check(lib().rtdbs_batch_slide_async(self.h, A, Lb, C))
+
+    def sync(self):
+        _check(lib().rtdbs_batch_sync(self.h))
+
+    def set_timing(self, enable=True):
+        _check(lib().rtdbs_batch_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        return _get_timing(lib().rtdbs_batch_get_timing, self.h)
+
+    def close(self):
+        if self.h:
+            lib().rtdbs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching slide server with HBM-budget admission."""
+
+    def __init__(self, hbm_budget_bytes, max_batch=64, lanes=2, use_graph=False, admit_wait_ms=0):
+        cfg = ServerConfig()
+        cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+        cfg.max_batch = int(max_batch)
+        cfg.lanes = int(lanes)
+        cfg.use_graph = int(use_graph)
+        cfg.admit_wait_ms = int(admit_wait_ms)
+        h = ctypes.c_void_p()
+        _check(lib().rtdbs_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h.value
+        self._keep = {}
+        self._meta = {}
+
+    def open_stream(self, dim, window, stride, eps, min_pts, first_window, lo=None, hi=None, cell_div=0):
+        p = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div)
+        pts = _f32(first_window, window, dim, "first window")
+        sid = ctypes.c_int32()
+        _check(lib().rtdbs_server_open_stream(self.h, ctypes.byref(p), _ptr(pts), ctypes.byref(sid)))
+        self._meta[sid.value] = (int(dim), int(window), int(stride))
+        return sid.value
+
+    def close_stream(self, sid):
+        _check(lib().rtdbs_server_close_stream(self.h, int(sid)))
+        self._meta.pop(sid, None)
+
+    def evict(self, sid):
+        _check(lib().rtdbs_server_evict_stream(self.h, int(sid)))
+
+    def readmit(self, sid):
+        _check(lib().rtdbs_server_readmit_stream(self.h, int(sid)))
+
+    def submit(self, sid, stride_points, label=None, cid=None):
+        dim, W, S = self._meta[sid]
+        pts = _f32(stride_points, S, dim, "stride")
+        if label is None:
+            label = np.empty(W, np.int32)
+        if cid is None:
+            cid = np.empty(W, np.int32)
+        t = ctypes.c_int64()
+        _check(lib().rtdbs_server_submit(self.h, int(sid), _ptr(pts), _ptr(label), _ptr(cid), ctypes.byref(t)))
+        self._keep[t.value] = (pts, label, cid)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        try:
+            _check(lib().rtdbs_server_wait(self.h, int(ticket), int(timeout_ms)))
+        except RtdbsError as e:
+            if e.code != ETIMEOUT:
+                self._keep.pop(ticket, None)
+            raise
+        _, label, cid = self._keep.pop(ticket)
+        return label, cid
+
+    def poll(self, ticket):
+        rc = lib().rtdbs_server_poll(self.h, int(ticket))
+        if rc < 0:
+            self._keep.pop(ticket, None)
+            _check(rc)
+        return rc == 1
+
+    def result(self, ticket):
+        _, label, cid = self._keep.pop(ticket)
+        return label, cid
+
+    def cancel(self, ticket):
+        _check(lib().rtdbs_server_cancel(self.h, int(ticket)))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().rtdbs_server_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def reset_stats(self):
+        _check(lib().rtdbs_server_reset_stats(self.h))
+
+    def close(self):
+        if self.h:
+            lib().rtdbs_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

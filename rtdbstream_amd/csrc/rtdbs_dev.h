@@ -1,0 +1,127 @@
+// Device-side layout shared by the HIP kernels and the C++ host engine.
+//
+// One stream's window lives in HBM sorted by the packed 64-bit key
+//     (cell_key << slot_bits) | slot
+// i.e. by (grid cell, ring slot) - the order of the reference's pos_arr
+// (thrust_helper.cu:25-27,42-46). Coordinates travel with the keys in SoA
+// arrays so every stencil row is a contiguous, coalesced range.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtdbs {
+
+constexpr int kTile = 2048;        // positions per tile: stride sort, merge, compaction
+constexpr int kTileThreads = 256;  // threads per tile block
+constexpr int kChunk = 64;         // positions per wave in the neighbourhood kernels
+constexpr int kCap = 512;          // candidates staged in LDS per wave
+constexpr int kMaxK = 3;           // largest stencil reach
+constexpr int kMaxRows = 49;       // (2k+1)^(dim-1) for k <= 3, dim <= 3
+constexpr int kKeyBits = 40;       // cell key bits; slot bits = 64 - 40 at most 24
+constexpr int kMaxSlotBits = 24;
+constexpr uint32_t kEpochMax = (1u << 24) - 1;  // hash epochs cycle 1..kEpochMax
+constexpr int kMaxBatch = 1024;
+constexpr int kNumCounters = 8;
+
+constexpr uint8_t kFlagCore = 1;
+constexpr uint8_t kFlagDense = 2;
+
+// Per-stream state that the device itself advances every slide.
+struct StreamMut {
+    int32_t t;       // slides completed
+    int32_t u;       // ring unit the next stride overwrites: slots [u*S, u*S+S)
+    uint32_t epoch;  // cell-hash epoch of the next slide
+    int32_t ncell;   // non-empty cells of the current window
+};
+
+// Per-slide I/O (device pointers).
+struct IO {
+    const float *in;  // S*dim floats (W*dim for the first window)
+    int32_t *label;   // W, slot-indexed
+    int32_t *cid;     // W, slot-indexed
+};
+
+struct StreamDev {
+    int32_t dim, W, S, min_pts;
+    int32_t k, R, slot_bits, units;
+    int32_t dense_ok;  // side eps/k with k*k > dim: points of one interior cell are within eps
+    float eps2;        // fl32(eps*eps)
+    uint32_t hmask;    // hash table size - 1
+    int32_t pad0;
+    double inv_side;
+    double lo[3];
+    int64_t n[3];  // grid cells per axis (1 for unused axes)
+    StreamMut *mut;
+    // sorted window (W)
+    uint64_t *skey;
+    float *sx, *sy, *sz;
+    // compacted survivors of the previous window (W), also init ping-pong
+    uint64_t *akey;
+    float *ax, *ay, *az;
+    // stride sort ping-pong (S)
+    uint64_t *bkey0;
+    float *bx0, *by0, *bz0;
+    uint64_t *bkey1;
+    float *bx1, *by1, *bz1;
+    // cell table
+    int32_t *cstart;  // W+1
+    uint64_t *ckey;   // W
+    int32_t *pcell;   // W: position -> cell
+    uint64_t *htab;   // cell hash: (epoch << 40) | cell_key
+    int32_t *hval;    // cell index
+    int2 *rows;       // W*R: candidate position ranges per cell
+    uint8_t *cdense;  // W: cell holds >= min_pts points that are all core
+    // clustering
+    uint8_t *flags;   // W: kFlagCore | kFlagDense
+    int32_t *parent;  // W: union-find over positions, root = minimum slot
+    int32_t *canon;   // W: canonical id (min core slot) of core positions
+    int32_t *tile_a;  // ceil(W/kTile): survivors per tile
+    int32_t *tile_b;  // ceil(W/kTile): cell starts per tile
+};
+
+struct BatchArgs {
+    const StreamDev *s;  // n descriptors
+    const IO *io;        // n I/O entries (unless use_io1)
+    IO io1;              // single-stream I/O passed by value
+    int32_t use_io1;
+    int32_t n;
+    const int32_t *pre_stile;  // n+1 prefix of ceil(S/kTile)
+    const int32_t *pre_wtile;  // n+1 prefix of ceil(W/kTile)
+    const int32_t *pre_chunk;  // n+1 prefix of ceil(W/kChunk)
+    int32_t *counters;         // kNumCounters work counters
+};
+
+// Kernel launchers (rtdbs_kernels.hip). All enqueue on `st` and return the
+// hipError_t of the launch.
+struct LaunchShape {
+    int32_t stiles;   // total stride tiles
+    int32_t wtiles;   // total window tiles
+    int32_t chunks;   // total chunks
+    int32_t passes;   // stride merge passes
+    int32_t persist_blocks;  // blocks for the persistent cell kernels
+};
+
+enum Phase {
+    kPhStrideSort = 0,
+    kPhMerge,
+    kPhExpire,
+    kPhMergeFinal,
+    kPhCells,
+    kPhRows,
+    kPhCount,
+    kPhUnion,
+    kPhFlatten,
+    kPhBorder,
+    kNumPhases
+};
+extern const char *const kPhaseNames[kNumPhases];
+
+// Enqueues one full slide for every stream of the batch. `ev` (optional,
+// 2*kNumPhases events) brackets each phase.
+hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev);
+// Enqueues the first-window load (sort of W points into the window arrays).
+hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipStream_t st);
+// Clears the cell hash of every stream in the batch (epoch wrap).
+hipError_t launch_clear_hash(const BatchArgs &a, hipStream_t st);
+
+}  // namespace rtdbs

@@ -1,0 +1,897 @@
+// Host engine of the MI355X streaming DBSCAN slide step: stream state,
+// plain (one stream, eager launches) and batched (many streams, one launch per
+// phase, optional HIP graph) slides, and the C ABI declared in include/rtdbs.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rtdbs.h"
+#include "rtdbs_dev.h"
+#include "rtdbs_host.h"
+
+using namespace rtdbs;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+
+namespace rtdbs {
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+}  // namespace rtdbs
+
+extern "C" const char *rtdbs_last_error(void) { return g_err.c_str(); }
+extern "C" const char *rtdbs_version(void) { return "rtdbs-mi355x 0.1 (gfx950)"; }
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(RTDBS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));  \
+    } while (0)
+
+namespace rtdbs {
+
+int device_cus() {
+    static int cus = 0;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    });
+    return cus;
+}
+
+// Unique across all streams, so a batch never mistakes a new stream that
+// reuses a freed stream's address for the old one.
+static uint64_t next_generation() {
+    static std::atomic<uint64_t> g{1};
+    return g.fetch_add(1);
+}
+
+static uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 64;
+    while (p < v) p <<= 1;
+    return (uint32_t)p;
+}
+
+static int ceil_log2(int64_t v) {
+    int b = 0;
+    while ((int64_t(1) << b) < v) b++;
+    return b;
+}
+
+static int default_cell_div(const rtdbs_params &p) {
+    // k = 2: cells of side eps/2 are dense-safe for dim <= 3 (k*k > dim), so a
+    // cell holding >= min_pts points is all core without counting.
+    (void)p;
+    return 2;
+}
+
+int validate_params(const rtdbs_params *p) {
+    if (!p) return fail(RTDBS_EINVAL, "params is NULL");
+    if (p->dim < 1 || p->dim > 3) return fail(RTDBS_EINVAL, "dim must be 1, 2 or 3 (got %d)", p->dim);
+    if (p->window < 1 || p->window > (1 << kMaxSlotBits))
+        return fail(RTDBS_EINVAL, "window must be in [1, %d] (got %d)", 1 << kMaxSlotBits, p->window);
+    if (p->stride < 1 || p->stride > p->window)
+        return fail(RTDBS_EINVAL, "stride must be in [1, window] (got %d)", p->stride);
+    if (p->window % p->stride != 0)
+        return fail(RTDBS_EINVAL, "stride %d must divide window %d (the ring of dbscan.cpp:2260,2289)",
+                    p->stride, p->window);
+    if (!(p->eps > 0.0f) || !std::isfinite(p->eps)) return fail(RTDBS_EINVAL, "eps must be finite and > 0");
+    if (p->min_pts < 1) return fail(RTDBS_EINVAL, "min_pts must be >= 1 (got %d)", p->min_pts);
+    if (p->cell_div < 0 || p->cell_div > kMaxK)
+        return fail(RTDBS_EINVAL, "cell_div must be 0 (auto) or 1..%d (got %d)", kMaxK, p->cell_div);
+    for (int a = 0; a < p->dim; a++)
+        if (!std::isfinite(p->lo[a]) || !std::isfinite(p->hi[a]))
+            return fail(RTDBS_EINVAL, "grid bounds must be finite");
+    return RTDBS_OK;
+}
+
+// Buffer layout of one stream's device arena.
+struct Layout {
+    size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
+    size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
+    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cdense, off_flags, off_parent,
+        off_canon, off_tile_a, off_tile_b, off_in, off_label, off_cid;
+    size_t total;
+};
+
+static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
+    Layout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += (bytes + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t W = (size_t)p.window, S = (size_t)p.stride, d = (size_t)p.dim;
+    const size_t ntile = (W + kTile - 1) / kTile;
+    L.off_mut = take(sizeof(StreamMut));
+    L.off_skey = take(W * 8);
+    L.off_sx = take(W * 4);
+    L.off_sy = d > 1 ? take(W * 4) : 0;
+    L.off_sz = d > 2 ? take(W * 4) : 0;
+    L.off_akey = take(W * 8);
+    L.off_ax = take(W * 4);
+    L.off_ay = d > 1 ? take(W * 4) : 0;
+    L.off_az = d > 2 ? take(W * 4) : 0;
+    L.off_bkey0 = take(S * 8);
+    L.off_bx0 = take(S * 4);
+    L.off_by0 = d > 1 ? take(S * 4) : 0;
+    L.off_bz0 = d > 2 ? take(S * 4) : 0;
+    L.off_bkey1 = take(S * 8);
+    L.off_bx1 = take(S * 4);
+    L.off_by1 = d > 1 ? take(S * 4) : 0;
+    L.off_bz1 = d > 2 ? take(S * 4) : 0;
+    L.off_cstart = take((W + 1) * 4);
+    L.off_ckey = take(W * 8);
+    L.off_pcell = take(W * 4);
+    L.off_htab = take((size_t)hsize * 8);
+    L.off_hval = take((size_t)hsize * 4);
+    L.off_rows = take(W * (size_t)R * 8);
+    L.off_cdense = take(W);
+    L.off_flags = take(W);
+    L.off_parent = take(W * 4);
+    L.off_canon = take(W * 4);
+    L.off_tile_a = take(ntile * 4);
+    L.off_tile_b = take(ntile * 4);
+    L.off_in = take(W * d * 4);
+    L.off_label = take(W * 4);
+    L.off_cid = take(W * 4);
+    L.total = o;
+    return L;
+}
+
+static void stream_shape(const rtdbs_params &p, int *k, int *R, uint32_t *hsize) {
+    *k = p.cell_div > 0 ? p.cell_div : default_cell_div(p);
+    int r = 1;
+    for (int a = 0; a < p.dim - 1; a++) r *= 2 * (*k) + 1;
+    *R = r;
+    *hsize = next_pow2(2ull * (uint64_t)p.window);
+}
+
+size_t stream_device_bytes(const rtdbs_params &p) {
+    int k, R;
+    uint32_t hs;
+    stream_shape(p, &k, &R, &hs);
+    return make_layout(p, R, hs).total;
+}
+
+}  // namespace rtdbs
+
+// ---------------------------------------------------------------------------
+// streams
+
+extern "C" int rtdbs_stream_device_bytes(const rtdbs_params *params, size_t *bytes) {
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (!bytes) return fail(RTDBS_EINVAL, "bytes is NULL");
+    *bytes = stream_device_bytes(*params);
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **out) {
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (!out) return fail(RTDBS_EINVAL, "out is NULL");
+    *out = nullptr;
+    rtdbs_stream *s = new (std::nothrow) rtdbs_stream();
+    if (!s) return fail(RTDBS_ENOMEM, "host allocation failed");
+    s->p = *params;
+    stream_shape(s->p, &s->k, &s->R, &s->hsize);
+    s->dense_ok = (s->k * s->k > s->p.dim) ? 1 : 0;
+    s->slot_bits = std::max(1, ceil_log2(s->p.window));
+    const Layout L = make_layout(s->p, s->R, s->hsize);
+    s->bytes = L.total;
+    if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
+    if (alloc_hook_should_fail(L.total)) {
+        delete s;
+        return fail(RTDBS_ENOMEM, "device allocation of %zu bytes refused (allocation hook)", L.total);
+    }
+    hipError_t e = hipMalloc((void **)&s->arena, L.total);
+    if (e != hipSuccess) {
+        delete s;
+        (void)hipGetLastError();
+        return fail(RTDBS_ENOMEM, "hipMalloc(%zu) failed: %s", L.total, hipGetErrorString(e));
+    }
+    // zeroed hash (epoch 0 entries are stale for every epoch >= 1) and state
+    e = hipMemset(s->arena, 0, L.total);
+    if (e != hipSuccess) {
+        (void)hipFree(s->arena);
+        delete s;
+        return fail(RTDBS_EHIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    char *A = s->arena;
+    StreamDev &D = s->dev;
+    std::memset(&D, 0, sizeof(D));
+    D.dim = s->p.dim;
+    D.W = s->p.window;
+    D.S = s->p.stride;
+    D.min_pts = s->p.min_pts;
+    D.k = s->k;
+    D.R = s->R;
+    D.slot_bits = s->slot_bits;
+    D.units = s->p.window / s->p.stride;
+    D.dense_ok = s->dense_ok;
+    {
+        volatile float e_ = s->p.eps;
+        D.eps2 = e_ * e_;  // fl32(eps*eps), dbscan.cpp:41
+    }
+    D.hmask = s->hsize - 1;
+    D.mut = (StreamMut *)(A + L.off_mut);
+    auto f = [&](size_t off, bool used) { return used ? (float *)(A + off) : (float *)nullptr; };
+    const int d = s->p.dim;
+    D.skey = (uint64_t *)(A + L.off_skey);
+    D.sx = f(L.off_sx, true);
+    D.sy = f(L.off_sy, d > 1);
+    D.sz = f(L.off_sz, d > 2);
+    D.akey = (uint64_t *)(A + L.off_akey);
+    D.ax = f(L.off_ax, true);
+    D.ay = f(L.off_ay, d > 1);
+    D.az = f(L.off_az, d > 2);
+    D.bkey0 = (uint64_t *)(A + L.off_bkey0);
+    D.bx0 = f(L.off_bx0, true);
+    D.by0 = f(L.off_by0, d > 1);
+    D.bz0 = f(L.off_bz0, d > 2);
+    D.bkey1 = (uint64_t *)(A + L.off_bkey1);
+    D.bx1 = f(L.off_bx1, true);
+    D.by1 = f(L.off_by1, d > 1);
+    D.bz1 = f(L.off_bz1, d > 2);
+    D.cstart = (int32_t *)(A + L.off_cstart);
+    D.ckey = (uint64_t *)(A + L.off_ckey);
+    D.pcell = (int32_t *)(A + L.off_pcell);
+    D.htab = (uint64_t *)(A + L.off_htab);
+    D.hval = (int32_t *)(A + L.off_hval);
+    D.rows = (int2 *)(A + L.off_rows);
+    D.cdense = (uint8_t *)(A + L.off_cdense);
+    D.flags = (uint8_t *)(A + L.off_flags);
+    D.parent = (int32_t *)(A + L.off_parent);
+    D.canon = (int32_t *)(A + L.off_canon);
+    D.tile_a = (int32_t *)(A + L.off_tile_a);
+    D.tile_b = (int32_t *)(A + L.off_tile_b);
+    s->d_in = (float *)(A + L.off_in);
+    s->d_label = (int32_t *)(A + L.off_label);
+    s->d_cid = (int32_t *)(A + L.off_cid);
+    *out = s;
+    return RTDBS_OK;
+}
+
+namespace rtdbs {
+
+// Grid geometry: side = eps/k * (1 + 2^-16). A pair accepted by the FP32 test
+// has true distance below eps*(1 + 2^-21), so on every axis the exact cell
+// indices differ by at most k; the double evaluation of the index adds at most
+// n*2^-52 <= 2^-22 cells for the per-axis caps below.
+static void set_geometry(rtdbs_stream *s, const float *pts) {
+    const rtdbs_params &p = s->p;
+    const int d = p.dim;
+    const int64_t cap = d == 3 ? (int64_t(1) << 13) : d == 2 ? (int64_t(1) << 20) : (int64_t(1) << 30);
+    s->side = (double)p.eps / (double)s->k * (1.0 + 1.0 / 65536.0);
+    const double inv = 1.0 / s->side;
+    bool given = true;
+    for (int a = 0; a < d; a++)
+        if (!(p.hi[a] > p.lo[a])) given = false;
+    for (int a = 0; a < 3; a++) {
+        s->lo[a] = 0.0;
+        s->n[a] = 1;
+    }
+    for (int a = 0; a < d; a++) {
+        double lo, hi;
+        if (given) {
+            lo = p.lo[a];
+            hi = p.hi[a];
+        } else {
+            float mn = INFINITY, mx = -INFINITY;
+            for (int i = 0; i < p.window; i++) {
+                const float v = pts[(size_t)i * d + a];
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+            if (!(mx >= mn)) mn = mx = 0.0f;  // all NaN
+            const double ext = (double)mx - (double)mn;
+            // room for the stream to drift; points outside are clamped into
+            // the border cells, which keeps the search exact
+            lo = (double)mn - 0.5 * ext - p.eps;
+            hi = (double)mx + 0.5 * ext + p.eps;
+            lo = (double)(float)lo;
+        }
+        s->lo[a] = lo;
+        double cells = std::floor((hi - lo) * inv) + 1.0;
+        if (!(cells >= 1.0)) cells = 1.0;
+        s->n[a] = cells > (double)cap ? cap : (int64_t)cells;
+    }
+    StreamDev &D = s->dev;
+    D.inv_side = inv;
+    for (int a = 0; a < 3; a++) {
+        D.lo[a] = s->lo[a];
+        D.n[a] = s->n[a];
+    }
+}
+
+}  // namespace rtdbs
+
+static int stream_load(rtdbs_stream *s, const float *points, int64_t slides_done) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    if (!points) return fail(RTDBS_EINVAL, "points is NULL");
+    if (slides_done < 0) return fail(RTDBS_EINVAL, "slides_done must be >= 0");
+    HIPCHK(hipSetDevice(s->device));
+    if (s->solo) {
+        int rc = batch_sync(s->solo);
+        if (rc) return rc;
+    }
+    set_geometry(s, points);
+    s->gen = next_generation();
+    int rc = ensure_solo(s);
+    if (rc) return rc;
+    rtdbs_batch *b = s->solo;
+    rc = refresh_batch(b);
+    if (rc) return rc;
+    const size_t W = (size_t)s->p.window, d = (size_t)s->p.dim;
+    const int units = s->p.window / s->p.stride;
+    HIPCHK(hipMemcpyAsync(s->d_in, points, W * d * 4, hipMemcpyHostToDevice, b->hs));
+    StreamMut m0 = {(int32_t)(slides_done % (1 << 30)), (int32_t)(slides_done % units), 1u, 0};
+    HIPCHK(hipMemcpyAsync(s->dev.mut, &m0, sizeof(m0), hipMemcpyHostToDevice, b->hs));
+    HIPCHK(hipMemsetAsync(s->dev.htab, 0, (size_t)s->hsize * 8, b->hs));
+    BatchArgs a = b->args;
+    a.use_io1 = 1;
+    a.io1.in = s->d_in;
+    a.io1.label = s->d_label;
+    a.io1.cid = s->d_cid;
+    const int itiles = (int)((W + kTile - 1) / kTile);
+    int passes = ceil_log2(itiles);
+    if (passes % 2 == 0) passes++;  // result must land in the window arrays
+    HIPCHK(launch_init(a, itiles, passes, b->hs));
+    HIPCHK(hipStreamSynchronize(b->hs));
+    s->initialized = true;
+    s->slides = slides_done;
+    s->epoch_base = slides_done;
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_stream_init(rtdbs_stream *s, const float *points) { return stream_load(s, points, 0); }
+
+extern "C" int rtdbs_stream_restore(rtdbs_stream *s, const float *window_points, int64_t slides_done) {
+    return stream_load(s, window_points, slides_done);
+}
+
+extern "C" int rtdbs_stream_slide(rtdbs_stream *s, const float *stride_points, int32_t *label, int32_t *cid) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    if (!stride_points || !label || !cid) return fail(RTDBS_EINVAL, "NULL buffer");
+    if (!s->initialized) return fail(RTDBS_ESTATE, "slide before rtdbs_stream_init");
+    HIPCHK(hipSetDevice(s->device));
+    rtdbs_batch *b = s->solo;
+    const size_t S = (size_t)s->p.stride, W = (size_t)s->p.window, d = (size_t)s->p.dim;
+    HIPCHK(hipMemcpyAsync(s->d_in, stride_points, S * d * 4, hipMemcpyHostToDevice, b->hs));
+    int rc = batch_launch_solo(b, s->d_in, s->d_label, s->d_cid);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(label, s->d_label, W * 4, hipMemcpyDeviceToHost, b->hs));
+    HIPCHK(hipMemcpyAsync(cid, s->d_cid, W * 4, hipMemcpyDeviceToHost, b->hs));
+    return batch_sync(b);
+}
+
+extern "C" int rtdbs_stream_slide_async(rtdbs_stream *s, const float *stride_points_dev, int32_t *label_dev,
+                                        int32_t *cid_dev) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    if (!stride_points_dev || !label_dev || !cid_dev) return fail(RTDBS_EINVAL, "NULL buffer");
+    if (!s->initialized) return fail(RTDBS_ESTATE, "slide before rtdbs_stream_init");
+    HIPCHK(hipSetDevice(s->device));
+    return batch_launch_solo(s->solo, stride_points_dev, label_dev, cid_dev);
+}
+
+extern "C" int rtdbs_stream_sync(rtdbs_stream *s) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    if (!s->solo) return RTDBS_OK;
+    HIPCHK(hipSetDevice(s->device));
+    return batch_sync(s->solo);
+}
+
+extern "C" int rtdbs_stream_get_info(const rtdbs_stream *s, rtdbs_stream_info *info) {
+    if (!s || !info) return fail(RTDBS_EINVAL, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->slides = s->slides;
+    info->cell_div = s->k;
+    info->rows = s->R;
+    info->dense_shortcut = s->dense_ok;
+    for (int a = 0; a < 3; a++) {
+        info->grid[a] = s->n[a];
+        info->lo[a] = s->lo[a];
+    }
+    info->cell_side = s->side;
+    info->device_bytes = s->bytes;
+    if (s->initialized) {
+        StreamMut m;
+        if (hipMemcpy(&m, s->dev.mut, sizeof(m), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(RTDBS_EHIP, "reading stream state failed");
+        info->cells = m.ncell;
+    }
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_stream_get_window(rtdbs_stream *s, float *points) {
+    if (!s || !points) return fail(RTDBS_EINVAL, "NULL argument");
+    if (!s->initialized) return fail(RTDBS_ESTATE, "stream not initialised");
+    HIPCHK(hipSetDevice(s->device));
+    if (s->solo) {
+        int rc = batch_sync(s->solo);
+        if (rc) return rc;
+    }
+    const size_t W = (size_t)s->p.window;
+    const int d = s->p.dim;
+    std::vector<uint64_t> key(W);
+    std::vector<float> c[3];
+    HIPCHK(hipMemcpy(key.data(), s->dev.skey, W * 8, hipMemcpyDeviceToHost));
+    const float *src[3] = {s->dev.sx, s->dev.sy, s->dev.sz};
+    for (int a = 0; a < d; a++) {
+        c[a].resize(W);
+        HIPCHK(hipMemcpy(c[a].data(), src[a], W * 4, hipMemcpyDeviceToHost));
+    }
+    const uint64_t m = (1ull << s->slot_bits) - 1ull;
+    for (size_t i = 0; i < W; i++) {
+        const size_t slot = (size_t)(key[i] & m);
+        if (slot >= W) return fail(RTDBS_EHIP, "corrupt window state");
+        for (int a = 0; a < d; a++) points[slot * d + a] = c[a][i];
+    }
+    return RTDBS_OK;
+}
+
+extern "C" void rtdbs_stream_destroy(rtdbs_stream *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->solo) {
+        batch_free(s->solo);
+        s->solo = nullptr;
+    }
+    if (s->arena) (void)hipFree(s->arena);
+    delete s;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+
+namespace rtdbs {
+
+static int batch_alloc(rtdbs_batch *b) {
+    const int n = b->capacity;
+    const size_t desc_b = ((size_t)n * sizeof(StreamDev) + 255) & ~size_t(255);
+    const size_t io_b = ((size_t)n * sizeof(IO) + 255) & ~size_t(255);
+    const size_t pre_b = ((size_t)3 * (n + 1) * 4 + 255) & ~size_t(255);
+    const size_t cnt_b = 256;
+    HIPCHK(hipMalloc((void **)&b->dmem, desc_b + io_b + pre_b + cnt_b));
+    b->d_desc = (StreamDev *)b->dmem;
+    b->d_io = (IO *)(b->dmem + desc_b);
+    b->d_pre = (int32_t *)(b->dmem + desc_b + io_b);
+    b->d_counters = (int32_t *)(b->dmem + desc_b + io_b + pre_b);
+    HIPCHK(hipHostMalloc((void **)&b->h_io, (size_t)n * sizeof(IO), hipHostMallocDefault));
+    HIPCHK(hipStreamCreateWithFlags(&b->hs, hipStreamNonBlocking));
+    for (int i = 0; i < 2 * kNumPhases; i++) HIPCHK(hipEventCreate(&b->ev[i]));
+    b->ev_ready = true;
+    return RTDBS_OK;
+}
+
+rtdbs_batch *batch_new(int capacity, bool use_graph, int *rc) {
+    rtdbs_batch *b = new (std::nothrow) rtdbs_batch();
+    if (!b) {
+        *rc = fail(RTDBS_ENOMEM, "host allocation failed");
+        return nullptr;
+    }
+    b->capacity = capacity;
+    b->use_graph = use_graph;
+    *rc = batch_alloc(b);
+    if (*rc) {
+        batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+int batch_set_members(rtdbs_batch *b, rtdbs_stream *const *streams, int n) {
+    if (n < 1 || n > b->capacity) return fail(RTDBS_EINVAL, "batch size %d outside [1, %d]", n, b->capacity);
+    bool same = n == b->n;
+    for (int i = 0; i < n && same; i++) same = b->streams[i] == streams[i] && b->gens[i] == streams[i]->gen;
+    if (same) return RTDBS_OK;
+    // the device may still read the old tables
+    if (b->hs) HIPCHK(hipStreamSynchronize(b->hs));
+    b->n = n;
+    b->streams.assign(streams, streams + n);
+    b->gens.assign(n, 0);
+    b->stage_io.resize(n);
+    for (int i = 0; i < n; i++) b->stage_io[i] = IO{streams[i]->d_in, streams[i]->d_label, streams[i]->d_cid};
+    return refresh_batch(b);
+}
+
+void batch_free(rtdbs_batch *b) {
+    if (!b) return;
+    if (b->hs) (void)hipStreamSynchronize(b->hs);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev_ready)
+        for (int i = 0; i < 2 * kNumPhases; i++)
+            if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+    if (b->hs) (void)hipStreamDestroy(b->hs);
+    if (b->h_io) (void)hipHostFree(b->h_io);
+    if (b->dmem) (void)hipFree(b->dmem);
+    delete b;
+}
+
+// Uploads descriptors and prefix tables; drops a captured graph. Called when
+// the batch is built and whenever a member stream was re-initialised.
+int refresh_batch(rtdbs_batch *b) {
+    const int n = b->n;
+    if (b->hs) HIPCHK(hipStreamSynchronize(b->hs));
+    std::vector<StreamDev> desc(n);
+    std::vector<int32_t> pre(3 * (n + 1), 0);
+    int passes = 0;
+    long long rows_total = 0, cells_total = 0;
+    for (int i = 0; i < n; i++) {
+        rtdbs_stream *s = b->streams[i];
+        desc[i] = s->dev;
+        const int st = (s->p.stride + kTile - 1) / kTile;
+        const int wt = (s->p.window + kTile - 1) / kTile;
+        const int ch = (s->p.window + kChunk - 1) / kChunk;
+        pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + st;
+        pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + wt;
+        pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + ch;
+        passes = std::max(passes, ceil_log2(st));
+        rows_total += (long long)s->p.window * s->R;
+        cells_total += s->p.window;
+        b->gens[i] = s->gen;
+    }
+    HIPCHK(hipMemcpy(b->d_desc, desc.data(), (size_t)n * sizeof(StreamDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_pre, pre.data(), pre.size() * 4, hipMemcpyHostToDevice));
+    b->shape.stiles = pre[0 * (n + 1) + n];
+    b->shape.wtiles = pre[1 * (n + 1) + n];
+    b->shape.chunks = pre[2 * (n + 1) + n];
+    b->shape.passes = passes;
+    const long long cap = (long long)device_cus() * 8;
+    long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
+    b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
+    (void)cells_total;
+    BatchArgs &a = b->args;
+    std::memset(&a, 0, sizeof(a));
+    a.s = b->d_desc;
+    a.io = b->d_io;
+    a.n = n;
+    a.pre_stile = b->d_pre;
+    a.pre_wtile = b->d_pre + (n + 1);
+    a.pre_chunk = b->d_pre + 2 * (n + 1);
+    a.counters = b->d_counters;
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->io_valid = false;
+    return RTDBS_OK;
+}
+
+static int batch_check_current(rtdbs_batch *b) {
+    for (int i = 0; i < b->n; i++) {
+        if (!b->streams[i]->initialized) return fail(RTDBS_ESTATE, "stream %d of the batch is not initialised", i);
+        if (b->gens[i] != b->streams[i]->gen) return refresh_batch(b);
+    }
+    return RTDBS_OK;
+}
+
+// Clears the cell hash of streams whose epoch is about to wrap.
+static int batch_epoch_wrap(rtdbs_batch *b) {
+    for (int i = 0; i < b->n; i++) {
+        rtdbs_stream *s = b->streams[i];
+        const int64_t since = s->slides - s->epoch_base;
+        if (since > 0 && since % (int64_t)kEpochMax == 0) {
+            int rc = ensure_solo(s);
+            if (rc) return rc;
+            BatchArgs a = s->solo->args;
+            HIPCHK(launch_clear_hash(a, b->hs));
+        }
+    }
+    return RTDBS_OK;
+}
+
+int ensure_solo(rtdbs_stream *s) {
+    if (s->solo) return RTDBS_OK;
+    int rc = 0;
+    rtdbs_batch *b = batch_new(1, false, &rc);
+    if (!b) return rc;
+    b->solo = true;
+    rtdbs_stream *one[1] = {s};
+    rc = batch_set_members(b, one, 1);
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    s->solo = b;
+    return RTDBS_OK;
+}
+
+static void timing_begin(rtdbs_batch *b) { (void)b; }
+
+int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *cid) {
+    int rc = batch_check_current(b);
+    if (rc) return rc;
+    rc = batch_epoch_wrap(b);
+    if (rc) return rc;
+    BatchArgs a = b->args;
+    a.use_io1 = 1;
+    a.io1.in = in;
+    a.io1.label = label;
+    a.io1.cid = cid;
+    if (b->timing && b->pending_timing) {
+        rc = batch_collect_timing(b);
+        if (rc) return rc;
+    }
+    timing_begin(b);
+    HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr));
+    b->pending_timing = b->timing;
+    for (auto *s : b->streams) s->slides++;
+    return RTDBS_OK;
+}
+
+int batch_collect_timing(rtdbs_batch *b) {
+    if (!b->pending_timing) return RTDBS_OK;
+    HIPCHK(hipEventSynchronize(b->ev[2 * kNumPhases - 1]));
+    for (int ph = 0; ph < kNumPhases; ph++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * ph], b->ev[2 * ph + 1]));
+        b->acc_ms[ph] += ms;
+        b->acc_n[ph]++;
+    }
+    b->pending_timing = false;
+    return RTDBS_OK;
+}
+
+int batch_sync(rtdbs_batch *b) {
+    HIPCHK(hipStreamSynchronize(b->hs));
+    return batch_collect_timing(b);
+}
+
+// One slide of all member streams with device I/O (table in d_io).
+int batch_launch(rtdbs_batch *b, const IO *io) {
+    int rc = batch_check_current(b);
+    if (rc) return rc;
+    rc = batch_epoch_wrap(b);
+    if (rc) return rc;
+    bool changed = !b->io_valid;
+    for (int i = 0; i < b->n && !changed; i++)
+        if (b->h_io[i].in != io[i].in || b->h_io[i].label != io[i].label || b->h_io[i].cid != io[i].cid)
+            changed = true;
+    if (changed) {
+        // the previous slide may still read the table
+        HIPCHK(hipStreamSynchronize(b->hs));
+        for (int i = 0; i < b->n; i++) b->h_io[i] = io[i];
+        HIPCHK(hipMemcpyAsync(b->d_io, b->h_io, (size_t)b->n * sizeof(IO), hipMemcpyHostToDevice, b->hs));
+        b->io_valid = true;
+    }
+    BatchArgs a = b->args;
+    a.use_io1 = 0;
+    if (b->use_graph && !b->timing) {
+        if (!b->gexec) {
+            HIPCHK(hipStreamBeginCapture(b->hs, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launch_slide(a, b->shape, b->hs, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(b->hs, &g);
+            if (le != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(RTDBS_EHIP, "graph capture launch failed: %s", hipGetErrorString(le));
+            }
+            if (ce != hipSuccess) return fail(RTDBS_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ce));
+            b->graph = g;
+            HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(b->gexec, b->hs));
+        b->pending_timing = false;
+    } else {
+        if (b->timing && b->pending_timing) {
+            rc = batch_collect_timing(b);
+            if (rc) return rc;
+        }
+        HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr));
+        b->pending_timing = b->timing;
+    }
+    for (auto *s : b->streams) s->slides++;
+    return RTDBS_OK;
+}
+
+}  // namespace rtdbs
+
+extern "C" int rtdbs_batch_create(rtdbs_stream *const *streams, int n, int use_graph, rtdbs_batch **out) {
+    if (!out) return fail(RTDBS_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!streams || n < 1 || n > kMaxBatch) return fail(RTDBS_EINVAL, "batch size must be in [1, %d]", kMaxBatch);
+    for (int i = 0; i < n; i++) {
+        if (!streams[i]) return fail(RTDBS_EINVAL, "stream %d is NULL", i);
+        if (!streams[i]->initialized) return fail(RTDBS_ESTATE, "stream %d is not initialised", i);
+        for (int j = 0; j < i; j++)
+            if (streams[j] == streams[i]) return fail(RTDBS_EINVAL, "stream %d appears twice", i);
+    }
+    int rc = 0;
+    rtdbs_batch *b = batch_new(n, use_graph != 0, &rc);
+    if (!b) return rc;
+    rc = batch_set_members(b, streams, n);
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_batch_slide(rtdbs_batch *b, const float *const *stride_points, int32_t *const *labels,
+                                 int32_t *const *cids) {
+    if (!b || !stride_points || !labels || !cids) return fail(RTDBS_EINVAL, "NULL argument");
+    for (int i = 0; i < b->n; i++)
+        if (!stride_points[i] || !labels[i] || !cids[i]) return fail(RTDBS_EINVAL, "NULL buffer for stream %d", i);
+    for (int i = 0; i < b->n; i++) {
+        rtdbs_stream *s = b->streams[i];
+        HIPCHK(hipMemcpyAsync(s->d_in, stride_points[i], (size_t)s->p.stride * s->p.dim * 4,
+                              hipMemcpyHostToDevice, b->hs));
+    }
+    int rc = batch_launch(b, b->stage_io.data());
+    if (rc) return rc;
+    for (int i = 0; i < b->n; i++) {
+        rtdbs_stream *s = b->streams[i];
+        HIPCHK(hipMemcpyAsync(labels[i], s->d_label, (size_t)s->p.window * 4, hipMemcpyDeviceToHost, b->hs));
+        HIPCHK(hipMemcpyAsync(cids[i], s->d_cid, (size_t)s->p.window * 4, hipMemcpyDeviceToHost, b->hs));
+    }
+    return batch_sync(b);
+}
+
+extern "C" int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *stride_points_dev,
+                                       int32_t *const *labels_dev, int32_t *const *cids_dev) {
+    if (!b || !stride_points_dev || !labels_dev || !cids_dev) return fail(RTDBS_EINVAL, "NULL argument");
+    std::vector<IO> io(b->n);
+    for (int i = 0; i < b->n; i++) {
+        if (!stride_points_dev[i] || !labels_dev[i] || !cids_dev[i])
+            return fail(RTDBS_EINVAL, "NULL buffer for stream %d", i);
+        io[i] = IO{stride_points_dev[i], labels_dev[i], cids_dev[i]};
+    }
+    return batch_launch(b, io.data());
+}
+
+extern "C" int rtdbs_batch_sync(rtdbs_batch *b) {
+    if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
+    return batch_sync(b);
+}
+
+extern "C" void rtdbs_batch_destroy(rtdbs_batch *b) { batch_free(b); }
+
+// ---------------------------------------------------------------------------
+// timing
+
+extern "C" int rtdbs_batch_set_timing(rtdbs_batch *b, int enable) {
+    if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    b->timing = enable != 0;
+    for (int i = 0; i < kNumPhases; i++) {
+        b->acc_ms[i] = 0.0;
+        b->acc_n[i] = 0;
+    }
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_stream_set_timing(rtdbs_stream *s, int enable) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    int rc = ensure_solo(s);
+    if (rc) return rc;
+    return rtdbs_batch_set_timing(s->solo, enable);
+}
+
+extern "C" int rtdbs_batch_get_timing(rtdbs_batch *b, double *ms, int64_t *launches, int max_phases, char *names,
+                                      size_t names_len) {
+    if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    const int np = std::min(max_phases, (int)kNumPhases);
+    for (int i = 0; i < np; i++) {
+        if (ms) ms[i] = b->acc_ms[i];
+        if (launches) launches[i] = b->acc_n[i];
+    }
+    if (names && names_len > 0) {
+        std::string all;
+        for (int i = 0; i < kNumPhases; i++) {
+            if (i) all += ",";
+            all += kPhaseNames[i];
+        }
+        std::snprintf(names, names_len, "%s", all.c_str());
+    }
+    return kNumPhases;
+}
+
+extern "C" int rtdbs_stream_get_timing(rtdbs_stream *s, double *ms, int64_t *launches, int max_phases, char *names,
+                                       size_t names_len) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    int rc = ensure_solo(s);
+    if (rc) return rc;
+    return rtdbs_batch_get_timing(s->solo, ms, launches, max_phases, names, names_len);
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+extern "C" int rtdbs_device_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(RTDBS_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RTDBS_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_device_free(void *ptr) {
+    HIPCHK(hipFree(ptr));
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes) {
+    HIPCHK(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice));
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes) {
+    HIPCHK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_device_synchronize(void) {
+    HIPCHK(hipDeviceSynchronize());
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_device_count(int *count) {
+    if (!count) return fail(RTDBS_EINVAL, "count is NULL");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+    }
+    return RTDBS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// test hook: host-forced allocation failures
+
+#include <atomic>
+
+static std::atomic<int> g_fail_allocs{0};
+
+namespace rtdbs {
+bool alloc_hook_should_fail(size_t bytes) {
+    (void)bytes;
+    int v = g_fail_allocs.load();
+    while (v > 0) {
+        if (g_fail_allocs.compare_exchange_weak(v, v - 1)) return true;
+    }
+    return false;
+}
+}  // namespace rtdbs
+
+extern "C" int rtdbs_test_fail_allocations(int count) {
+    g_fail_allocs.store(count > 0 ? count : 0);
+    return RTDBS_OK;
+}

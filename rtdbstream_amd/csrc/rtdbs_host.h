@@ -1,0 +1,81 @@
+// Host-side objects behind the opaque handles of include/rtdbs.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "rtdbs.h"
+#include "rtdbs_dev.h"
+
+struct rtdbs_batch;
+
+struct rtdbs_stream {
+    rtdbs_params p{};
+    int k = 0, R = 0, slot_bits = 0, dense_ok = 0;
+    uint32_t hsize = 0;
+    double side = 0.0;
+    double lo[3] = {0, 0, 0};
+    int64_t n[3] = {1, 1, 1};
+    size_t bytes = 0;
+    char *arena = nullptr;
+    rtdbs::StreamDev dev{};   // host copy of the device descriptor
+    float *d_in = nullptr;     // stride / first-window staging (device)
+    int32_t *d_label = nullptr;  // output staging for host-buffer slides
+    int32_t *d_cid = nullptr;
+    bool initialized = false;
+    int64_t slides = 0;
+    int64_t epoch_base = 0;    // value of `slides` when the cell hash was last cleared
+    uint64_t gen = 0;          // bumped by every init: batches re-read descriptors
+    rtdbs_batch *solo = nullptr;  // single-stream batch of the plain path
+    int device = 0;
+};
+
+struct rtdbs_batch {
+    int n = 0;
+    int capacity = 0;
+    bool solo = false;
+    bool use_graph = false;
+    std::vector<rtdbs_stream *> streams;
+    std::vector<uint64_t> gens;
+    std::vector<rtdbs::IO> stage_io;
+    char *dmem = nullptr;
+    rtdbs::StreamDev *d_desc = nullptr;
+    rtdbs::IO *d_io = nullptr;
+    int32_t *d_pre = nullptr;
+    int32_t *d_counters = nullptr;
+    rtdbs::IO *h_io = nullptr;  // pinned mirror of d_io
+    bool io_valid = false;
+    rtdbs::BatchArgs args{};
+    rtdbs::LaunchShape shape{};
+    hipStream_t hs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool timing = false;
+    bool pending_timing = false;
+    bool ev_ready = false;
+    hipEvent_t ev[2 * rtdbs::kNumPhases] = {};
+    double acc_ms[rtdbs::kNumPhases] = {};
+    int64_t acc_n[rtdbs::kNumPhases] = {};
+};
+
+namespace rtdbs {
+int fail(int code, const char *fmt, ...);
+int validate_params(const rtdbs_params *p);
+size_t stream_device_bytes(const rtdbs_params &p);
+int device_cus();
+int ensure_solo(rtdbs_stream *s);
+int refresh_batch(rtdbs_batch *b);
+int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *cid);
+int batch_launch(rtdbs_batch *b, const IO *io);
+int batch_sync(rtdbs_batch *b);
+int batch_collect_timing(rtdbs_batch *b);
+void batch_free(rtdbs_batch *b);
+// Allocates a batch able to hold up to `capacity` streams (members set later).
+rtdbs_batch *batch_new(int capacity, bool use_graph, int *rc);
+// Replaces the member streams (n <= capacity); drops any captured graph
+// unless the membership is unchanged.
+int batch_set_members(rtdbs_batch *b, rtdbs_stream *const *streams, int n);
+// Test hook: makes the next stream allocations fail (host-forced ENOMEM).
+bool alloc_hook_should_fail(size_t bytes);
+}  // namespace rtdbs

@@ -1,0 +1,124 @@
+"""Parity of the HIP engine with the oracle, through the C ABI.
+
+Bar: bit-exact `label` and `cid` on every slot of every slide (integer
+output; the FP32 distance test is evaluated in the same pinned order on both
+sides). Inputs are seeded; sizes stay where the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+from rtdbstream_amd import workloads as WL
+from tests.refcheck import reference_check, window_at
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_slides(R, data, dim, W, S, eps, mp, slides, cell_div=0, lo=None, hi=None):
+    s = R.Stream(dim, W, S, eps, mp, lo=lo, hi=hi, cell_div=cell_div)
+    s.init(data[:W])
+    out = []
+    for t in range(slides):
+        l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        out.append((l.copy(), c.copy()))
+    s.close()
+    return out
+
+
+def assert_parity(got, ref_labels, ref_cids, what):
+    for t, (l, c) in enumerate(got):
+        if not np.array_equal(l, ref_labels[t]):
+            i = int(np.flatnonzero(l != ref_labels[t])[0])
+            pytest.fail("%s slide %d: label[%d] = %d, oracle %d" % (what, t, i, l[i], ref_labels[t][i]))
+        if not np.array_equal(c, ref_cids[t]):
+            i = int(np.flatnonzero(c != ref_cids[t])[0])
+            pytest.fail("%s slide %d: cid[%d] = %d, oracle %d (label %d)" % (what, t, i, c[i], ref_cids[t][i], l[i]))
+
+
+def _mixed(n, dim, seed, spread=6.0, sigma=0.25, noise=0.3):
+    rng = np.random.default_rng(seed)
+    nb = int(n * (1 - noise))
+    c = rng.uniform(0, spread, (6, dim))
+    p = c[rng.integers(0, 6, nb)] + rng.normal(0, sigma, (nb, dim))
+    q = rng.uniform(0, spread, (n - nb, dim))
+    allp = np.concatenate([p, q]).astype(np.float32)
+    return allp[rng.permutation(n)]
+
+
+SMALL = [
+    # dim, W, S, eps, min_pts, cell_div, seed
+    (2, 512, 64, 0.2, 4, 0, 1),
+    (2, 512, 512, 0.2, 4, 0, 2),     # S == W: whole window replaced
+    (3, 512, 128, 0.4, 5, 0, 3),
+    (1, 256, 32, 0.01, 3, 0, 4),
+    (2, 2048, 256, 0.15, 6, 1, 5),   # k = 1: no dense shortcut
+    (2, 2048, 256, 0.15, 6, 2, 6),
+    (2, 2048, 256, 0.15, 6, 3, 7),
+    (3, 2048, 512, 0.3, 10, 1, 8),
+    (3, 2048, 512, 0.3, 10, 2, 9),
+    (3, 2048, 512, 0.3, 10, 3, 10),
+    (2, 1000, 100, 0.3, 1, 0, 11),   # min_pts 1: everything core
+    (3, 1000, 250, 0.3, 2000, 0, 12),  # min_pts > W: all noise
+    (2, 4096, 2048, 0.1, 5, 2, 13),  # stride sort across tiles
+    (3, 6000, 3000, 0.25, 8, 2, 14), # merge passes
+]
+
+
+@pytest.mark.parametrize("dim,W,S,eps,mp,k,seed", SMALL)
+def test_small_streams_bit_exact(engine, oracle, dim, W, S, eps, mp, k, seed):
+    slides = 6
+    data = _mixed(W + slides * S, dim, seed)
+    ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
+    got = engine_slides(engine, data, dim, W, S, eps, mp, slides, cell_div=k)
+    assert_parity(got, ref_l, ref_c, "dim%d W%d S%d k%d" % (dim, W, S, k))
+
+
+def test_lattice_ties(engine, oracle):
+    # points on a 0.5 lattice with eps = 0.5: every nearest pair is an exact tie
+    rng = np.random.default_rng(21)
+    W, S, slides = 1024, 128, 5
+    data = (rng.integers(0, 24, (W + slides * S, 2)).astype(np.float32) * np.float32(0.5))
+    for mp in (2, 3, 5):
+        for k in (1, 2, 3):
+            ref_l, ref_c = oracle.stream(data, W, S, 0.5, mp, slides, method="grid")
+            got = engine_slides(engine, data, 2, W, S, 0.5, mp, slides, cell_div=k)
+            assert_parity(got, ref_l, ref_c, "lattice mp%d k%d" % (mp, k))
+
+
+def test_duplicate_points(engine, oracle):
+    # many identical points (d2 = 0) and a dense cell far above min_pts
+    rng = np.random.default_rng(22)
+    W, S, slides = 800, 200, 4
+    base = rng.uniform(0, 2, (40, 3)).astype(np.float32)
+    data = base[rng.integers(0, 40, W + slides * S)]
+    ref_l, ref_c = oracle.stream(data, W, S, 0.3, 6, slides, method="grid")
+    got = engine_slides(engine, data, 3, W, S, 0.3, 6, slides)
+    assert_parity(got, ref_l, ref_c, "duplicates")
+
+
+def test_out_of_bounds_points_clamped_exactly(engine, oracle):
+    # explicit grid bounds much smaller than the data: points outside are
+    # clamped into border cells, which must not change any label
+    data = _mixed(1200 + 5 * 300, 2, 23, spread=10.0)
+    ref_l, ref_c = oracle.stream(data, 1200, 300, 0.35, 4, 5, method="grid")
+    got = engine_slides(engine, data, 2, 1200, 300, 0.35, 4, 5, lo=[4.0, 4.0], hi=[5.0, 5.0])
+    assert_parity(got, ref_l, ref_c, "clamped")
+
+
+def test_blobs2d_full_config(engine, oracle):
+    cfg = WL.WORKLOADS["blobs2d"]
+    data = WL.generate("blobs2d")
+    W, S, n = cfg["window"], cfg["stride"], cfg["slides"]
+    ref_l, ref_c = oracle.stream(data, W, S, cfg["eps"], cfg["min_pts"], n, method="grid")
+    got = engine_slides(engine, data, 2, W, S, cfg["eps"], cfg["min_pts"], n)
+    assert_parity(got, ref_l, ref_c, "blobs2d")
+    # the reference's own acceptance rules on the last window
+    win = window_at(data, W, S, n - 1)
+    assert reference_check(win, cfg["eps"], got[-1][0], got[-1][1], ref_l[-1], ref_c[-1]) == []
+
+
+def test_repeat_runs_identical(engine):
+    data = _mixed(3000 + 4 * 500, 3, 31)
+    a = engine_slides(engine, data, 3, 3000, 500, 0.3, 6, 4)
+    b = engine_slides(engine, data, 3, 3000, 500, 0.3, 6, 4)
+    for (l1, c1), (l2, c2) in zip(a, b):
+        assert np.array_equal(l1, l2) and np.array_equal(c1, c2)
