@@ -48,7 +48,8 @@ def plain_leg(R, WL, name, steps, warmup):
     cfg = WL.WORKLOADS[name]
     W, S, dim, eps, mp = cfg["window"], cfg["stride"], cfg["dim"], cfg["eps"], cfg["min_pts"]
     total_slides = warmup + steps
-    data = WL.generate(name, n=W + total_slides * S)
+    extra = 10
+    data = WL.generate(name, n=W + (total_slides + extra) * S)
     d_data = R.DeviceBuffer(data.nbytes)
     d_data.upload(data)
     d_label = R.DeviceBuffer(W * 4)
@@ -62,7 +63,7 @@ def plain_leg(R, WL, name, steps, warmup):
 
     for t in range(warmup):
         slide(t)
-    s.set_timing(True)
+    s.set_timing(2)  # events around the range-count kernel only
     lat = []
     R.lib().rtdbs_device_synchronize()
     t0 = time.perf_counter()
@@ -74,8 +75,15 @@ def plain_leg(R, WL, name, steps, warmup):
     elapsed = time.perf_counter() - t0
     timing = s.timing()
     info = s.info()
+    # per-phase breakdown from a separate pass (events between every phase
+    # add launch gaps, so they stay out of the timed region)
+    s.set_timing(1)
+    for t in range(total_slides, total_slides + min(steps, extra)):
+        slide(t)
+    phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in s.timing().items()}
     s.close()
-    return dict(data=data, cfg=cfg, elapsed=elapsed, lat=lat, timing=timing, info=info, W=W, S=S, dim=dim,
+    return dict(data=data, cfg=cfg, cfg_name=name, elapsed=elapsed, lat=lat, timing=timing, info=info, W=W, S=S, dim=dim,
+                phase_ms=phase_ms,
                 eps=eps, min_pts=mp, warmup=warmup, steps=steps)
 
 
@@ -103,20 +111,27 @@ def roofline(WL, res, sample=8):
             "avg_launch_us": round(avg_s * 1e6, 2)}
 
 
-def cpu_baseline(R, WL, res, seconds):
-    """CPU port (oracle grid, OpenMP) on a bounded sample of the same slides;
-    the GPU labels of those slides are recomputed and compared bit for bit."""
+def cpu_baseline(R, WL, res, seconds, max_slides=400):
+    """CPU port (oracle grid, OpenMP) on a bounded sample of the same workload
+    (about `seconds` of CPU work); the GPU labels of the same slides are
+    computed alongside and compared bit for bit."""
     from oracle import oracle as O
 
-    data, W, S, dim, eps, mp = res["data"], res["W"], res["S"], res["dim"], res["eps"], res["min_pts"]
+    W, S, dim, eps, mp = res["W"], res["S"], res["dim"], res["eps"], res["min_pts"]
+    name = res["cfg_name"]
+    data = WL.generate(name, n=W + max_slides * S)
     threads = O.max_threads()
     s = R.Stream(dim, W, S, eps, mp)
     s.init(data[:W])
+    win = np.array(data[:W], copy=True)
+    units = W // S
     n_done, cpu_time, exact = 0, 0.0, True
     t = 0
-    while (cpu_time < seconds or n_done < 2) and t < res["warmup"] + res["steps"]:
-        label, cid = s.slide(data[W + t * S: W + (t + 1) * S])
-        win = window_at(data, W, S, t)
+    while (cpu_time < seconds or n_done < 2) and t < max_slides:
+        stride = data[W + t * S: W + (t + 1) * S]
+        label, cid = s.slide(stride)
+        u = t % units
+        win[u * S:(u + 1) * S] = stride
         a = time.perf_counter()
         rl, rc = O.grid(win, eps, mp, threads=threads)
         cpu_time += time.perf_counter() - a
@@ -125,7 +140,8 @@ def cpu_baseline(R, WL, res, seconds):
         t += 1
     s.close()
     return {"value": round(n_done * S / cpu_time, 1), "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": "%d slides of %s (W=%d, S=%d), oracle grid + OpenMP, %.1f s" % (n_done, "window", W, S, cpu_time),
+            "sample": "%d slides of %s (W=%d, S=%d): oracle grid restatement, OpenMP on %d threads, %.1f s"
+                      % (n_done, name, W, S, threads, cpu_time),
             "slides_per_s": round(n_done / cpu_time, 3)}, exact, n_done
 
 
@@ -262,7 +278,7 @@ def main():
         "window_points_per_s": round(args.steps * W / res["elapsed"], 1),
         "p50_ms": round(float(np.percentile(res["lat"], 50)), 4),
         "p99_ms": round(float(np.percentile(res["lat"], 99)), 4),
-        "phase_ms": {k: round(v[0] / max(1, v[1]), 4) for k, v in res["timing"].items()},
+        "phase_ms": res["phase_ms"],
         "grid": {"cell_div": res["info"]["cell_div"], "cells_last": res["info"]["cells"]},
     }
     log("plain: %.1f points/s, %.3f ms/slide" % (out["value"], ms_per))

@@ -192,7 +192,8 @@ int rtdbs_device_synchronize(void);
 int rtdbs_device_count(int *count);
 
 /* ---- kernel timing (HIP events on the engine's own streams) -------------- */
-/* Enables per-phase event timing on a batch (or a stream's solo batch). */
+/* Event timing on a batch (or a stream's solo batch): enable = 0 off,
+ * 1 every phase, 2 only the range-count kernel (two events per slide). */
 int rtdbs_batch_set_timing(rtdbs_batch *b, int enable);
 int rtdbs_stream_set_timing(rtdbs_stream *s, int enable);
 /* Accumulated milliseconds and launch counts per phase since the last reset.

@@ -20,7 +20,9 @@ __all__ = [
 ]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_PKG, "librtdbs.so")
+# RTDBS_LIBRARY may point at another build of the same library (e.g. a
+# profiling build); the default is the in-tree one.
+_LIB_PATH = os.environ.get("RTDBS_LIBRARY") or os.path.join(_PKG, "librtdbs.so")
 
 LABEL_CORE, LABEL_BORDER, LABEL_NOISE = 0, 1, 2
 

@@ -25,6 +25,10 @@ constexpr int kNumCounters = 8;
 
 constexpr uint8_t kFlagCore = 1;
 constexpr uint8_t kFlagDense = 2;
+constexpr uint8_t kCellSafe = 1;
+constexpr uint8_t kCellDense = 2;
+constexpr int kBoxMin = 16;  // cells at least this large get a bounding box
+constexpr int kNoCore = 0x7fffffff;  // fcore of a cell without cores
 
 // Per-stream state that the device itself advances every slide.
 struct StreamMut {
@@ -32,6 +36,7 @@ struct StreamMut {
     int32_t u;       // ring unit the next stride overwrites: slots [u*S, u*S+S)
     uint32_t epoch;  // cell-hash epoch of the next slide
     int32_t ncell;   // non-empty cells of the current window
+    int32_t pad[4];
 };
 
 // Per-slide I/O (device pointers).
@@ -70,7 +75,9 @@ struct StreamDev {
     uint64_t *htab;   // cell hash: (epoch << 40) | cell_key
     int32_t *hval;    // cell index
     int2 *rows;       // W*R: candidate position ranges per cell
-    uint8_t *cdense;  // W: cell holds >= min_pts points that are all core
+    uint8_t *cflag;   // W: kCellSafe | kCellDense per cell
+    int32_t *fcore;   // W: first (minimum-slot) core position of each cell, kNoCore if none
+    float *cbox;      // 6W: per-cell bounding box (lo xyz, hi xyz) for cells of >= kBoxMin points
     // clustering
     uint8_t *flags;   // W: kFlagCore | kFlagDense
     int32_t *parent;  // W: union-find over positions, root = minimum slot
@@ -89,6 +96,7 @@ struct BatchArgs {
     const int32_t *pre_wtile;  // n+1 prefix of ceil(W/kTile)
     const int32_t *pre_chunk;  // n+1 prefix of ceil(W/kChunk)
     int32_t *counters;         // kNumCounters work counters
+    int32_t *cell_pre;         // n+1 prefix of cells per stream (written by k_rows)
 };
 
 // Kernel launchers (rtdbs_kernels.hip). All enqueue on `st` and return the
@@ -98,7 +106,8 @@ struct LaunchShape {
     int32_t wtiles;   // total window tiles
     int32_t chunks;   // total chunks
     int32_t passes;   // stride merge passes
-    int32_t persist_blocks;  // blocks for the persistent cell kernels
+    int32_t persist_blocks;  // 256-thread blocks for the persistent row kernel
+    int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
 };
 
 enum Phase {
@@ -117,8 +126,9 @@ enum Phase {
 extern const char *const kPhaseNames[kNumPhases];
 
 // Enqueues one full slide for every stream of the batch. `ev` (optional,
-// 2*kNumPhases events) brackets each phase.
-hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev);
+// 2*kNumPhases events) brackets each phase selected by phase_mask.
+hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev,
+                        uint32_t phase_mask);
 // Enqueues the first-window load (sort of W points into the window arrays).
 hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipStream_t st);
 // Clears the cell hash of every stream in the batch (epoch wrap).

@@ -111,7 +111,7 @@ int validate_params(const rtdbs_params *p) {
 struct Layout {
     size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
     size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
-    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cdense, off_flags, off_parent,
+    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cflag, off_fcore, off_cbox, off_flags, off_parent,
         off_canon, off_tile_a, off_tile_b, off_in, off_label, off_cid;
     size_t total;
 };
@@ -149,7 +149,9 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
     L.off_htab = take((size_t)hsize * 8);
     L.off_hval = take((size_t)hsize * 4);
     L.off_rows = take(W * (size_t)R * 8);
-    L.off_cdense = take(W);
+    L.off_cflag = take(W);
+    L.off_fcore = take(W * 4);
+    L.off_cbox = take(W * 24);
     L.off_flags = take(W);
     L.off_parent = take(W * 4);
     L.off_canon = take(W * 4);
@@ -263,7 +265,9 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     D.htab = (uint64_t *)(A + L.off_htab);
     D.hval = (int32_t *)(A + L.off_hval);
     D.rows = (int2 *)(A + L.off_rows);
-    D.cdense = (uint8_t *)(A + L.off_cdense);
+    D.cflag = (uint8_t *)(A + L.off_cflag);
+    D.fcore = (int32_t *)(A + L.off_fcore);
+    D.cbox = (float *)(A + L.off_cbox);
     D.flags = (uint8_t *)(A + L.off_flags);
     D.parent = (int32_t *)(A + L.off_parent);
     D.canon = (int32_t *)(A + L.off_canon);
@@ -349,7 +353,7 @@ static int stream_load(rtdbs_stream *s, const float *points, int64_t slides_done
     const size_t W = (size_t)s->p.window, d = (size_t)s->p.dim;
     const int units = s->p.window / s->p.stride;
     HIPCHK(hipMemcpyAsync(s->d_in, points, W * d * 4, hipMemcpyHostToDevice, b->hs));
-    StreamMut m0 = {(int32_t)(slides_done % (1 << 30)), (int32_t)(slides_done % units), 1u, 0};
+    StreamMut m0 = {(int32_t)(slides_done % (1 << 30)), (int32_t)(slides_done % units), 1u, 0, {0, 0, 0, 0}};
     HIPCHK(hipMemcpyAsync(s->dev.mut, &m0, sizeof(m0), hipMemcpyHostToDevice, b->hs));
     HIPCHK(hipMemsetAsync(s->dev.htab, 0, (size_t)s->hsize * 8, b->hs));
     BatchArgs a = b->args;
@@ -476,7 +480,9 @@ static int batch_alloc(rtdbs_batch *b) {
     const size_t io_b = ((size_t)n * sizeof(IO) + 255) & ~size_t(255);
     const size_t pre_b = ((size_t)3 * (n + 1) * 4 + 255) & ~size_t(255);
     const size_t cnt_b = 256;
-    HIPCHK(hipMalloc((void **)&b->dmem, desc_b + io_b + pre_b + cnt_b));
+    const size_t cpre_b = ((size_t)(n + 1) * 4 + 255) & ~size_t(255);
+    HIPCHK(hipMalloc((void **)&b->dmem, desc_b + io_b + pre_b + cnt_b + cpre_b));
+    b->d_cell_pre = (int32_t *)(b->dmem + desc_b + io_b + pre_b + cnt_b);
     b->d_desc = (StreamDev *)b->dmem;
     b->d_io = (IO *)(b->dmem + desc_b);
     b->d_pre = (int32_t *)(b->dmem + desc_b + io_b);
@@ -565,6 +571,7 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
+    b->shape.cell_blocks = (int)std::max(1ll, std::min((long long)device_cus() * 32, cells_total));
     (void)cells_total;
     BatchArgs &a = b->args;
     std::memset(&a, 0, sizeof(a));
@@ -575,6 +582,7 @@ int refresh_batch(rtdbs_batch *b) {
     a.pre_wtile = b->d_pre + (n + 1);
     a.pre_chunk = b->d_pre + 2 * (n + 1);
     a.counters = b->d_counters;
+    a.cell_pre = b->d_cell_pre;
     if (b->gexec) {
         (void)hipGraphExecDestroy(b->gexec);
         b->gexec = nullptr;
@@ -643,7 +651,7 @@ int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *
         if (rc) return rc;
     }
     timing_begin(b);
-    HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr));
+    HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr, b->timing_mask));
     b->pending_timing = b->timing;
     for (auto *s : b->streams) s->slides++;
     return RTDBS_OK;
@@ -651,8 +659,13 @@ int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *
 
 int batch_collect_timing(rtdbs_batch *b) {
     if (!b->pending_timing) return RTDBS_OK;
-    HIPCHK(hipEventSynchronize(b->ev[2 * kNumPhases - 1]));
+    for (int ph = kNumPhases - 1; ph >= 0; ph--)
+        if ((b->timing_mask >> ph) & 1u) {
+            HIPCHK(hipEventSynchronize(b->ev[2 * ph + 1]));
+            break;
+        }
     for (int ph = 0; ph < kNumPhases; ph++) {
+        if (!((b->timing_mask >> ph) & 1u)) continue;
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * ph], b->ev[2 * ph + 1]));
         b->acc_ms[ph] += ms;
@@ -689,7 +702,7 @@ int batch_launch(rtdbs_batch *b, const IO *io) {
     if (b->use_graph && !b->timing) {
         if (!b->gexec) {
             HIPCHK(hipStreamBeginCapture(b->hs, hipStreamCaptureModeThreadLocal));
-            hipError_t le = launch_slide(a, b->shape, b->hs, nullptr);
+            hipError_t le = launch_slide(a, b->shape, b->hs, nullptr, 0u);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(b->hs, &g);
             if (le != hipSuccess) {
@@ -707,7 +720,7 @@ int batch_launch(rtdbs_batch *b, const IO *io) {
             rc = batch_collect_timing(b);
             if (rc) return rc;
         }
-        HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr));
+        HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr, b->timing_mask));
         b->pending_timing = b->timing;
     }
     for (auto *s : b->streams) s->slides++;
@@ -784,7 +797,9 @@ extern "C" int rtdbs_batch_set_timing(rtdbs_batch *b, int enable) {
     if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
     int rc = batch_sync(b);
     if (rc) return rc;
+    // 1: every phase; 2: the range-count kernel only (two events per slide)
     b->timing = enable != 0;
+    b->timing_mask = enable == 2 ? (1u << kPhCount) : (enable ? (1u << kNumPhases) - 1u : 0u);
     for (int i = 0; i < kNumPhases; i++) {
         b->acc_ms[i] = 0.0;
         b->acc_n[i] = 0;

@@ -44,6 +44,7 @@ struct rtdbs_batch {
     rtdbs::IO *d_io = nullptr;
     int32_t *d_pre = nullptr;
     int32_t *d_counters = nullptr;
+    int32_t *d_cell_pre = nullptr;
     rtdbs::IO *h_io = nullptr;  // pinned mirror of d_io
     bool io_valid = false;
     rtdbs::BatchArgs args{};
@@ -52,6 +53,7 @@ struct rtdbs_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
+    uint32_t timing_mask = 0;  // phases bracketed by events
     bool pending_timing = false;
     bool ev_ready = false;
     hipEvent_t ev[2 * rtdbs::kNumPhases] = {};

@@ -76,6 +76,12 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
+// Value of lane `l` (uniform) of a per-lane float/int.
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 // Sum over a 256-thread block.
 __device__ __forceinline__ int block_sum(int v, int *lds4) {
     v = wave_sum(v);
@@ -144,47 +150,69 @@ __device__ void block_stream_prefix(int n, long long *pre, long long *tmp4, F va
     __syncthreads();
 }
 
-// Union-find over window positions. Loads bypass the non-coherent vector L1
-// (agent-scope relaxed atomics) so a walk sees the hooks other CUs made.
+// Union-find parent reads may be stale (served by a CU's L1 after another CU
+// lowered an entry): a stale value is still an ancestor, and hooks go through
+// L2 atomics, so stale reads cost extra steps, never correctness. Cacheable
+// loads keep the hot root entries of large clusters from funnelling every walk
+// into one L2 line.
 __device__ __forceinline__ int uf_load(const int32_t *p) {
+#ifdef RTDBS_UF_COHERENT_LOADS
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void uf_store(int32_t *p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    return *p;
+#endif
 }
 
-// Root of x with path halving. Every stored value is an ancestor of the old
-// one and roots only change through the CAS in uf_unite, so concurrent
-// halving is benign (the reference compresses the same way, dbscan.cu:9-20).
+// Union-find over ring slots, parent[x] <= x for every slot, so the root of a
+// component is its minimum slot - the reference's canonical cluster id (it
+// hooks the larger root under the smaller, dbscan.cu:22-43).
+//
+// Hooking uses atomicMin, which never fails: when b turned out not to be a
+// root any more, min(old, a) is stored and the caller goes on uniting a with
+// the displaced parent `old`, so no link is lost and b strictly decreases.
+// (A compare-and-swap hook fails and retries whenever another lane hooked the
+// same root first, which serialises every edge of a large cluster on its
+// root.) Path halving also lowers entries with atomicMin, so every entry only
+// ever decreases and always points at an ancestor.
 __device__ __forceinline__ int uf_find(int32_t *parent, int x) {
     int p = uf_load(parent + x);
     while (p != x) {
-        int gp = uf_load(parent + p);
-        if (gp != p) uf_store(parent + x, gp);
+        const int gp = uf_load(parent + p);
+#ifndef RTDBS_UF_NO_HALVING
+#ifdef RTDBS_UF_HALVE_STORE
+        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+        if (gp != p) atomicMin(parent + x, gp);
+#endif
+#endif
         x = p;
         p = gp;
     }
     return x;
 }
 
-// Hooks the root with the larger slot under the root with the smaller slot
-// (dbscan.cu:22-43 hooks by index; here the order is the ring slot, so the
-// final root of every component is its minimum slot).
-__device__ __forceinline__ void uf_unite(int32_t *parent, const uint64_t *skey, uint32_t smask, int a,
-                                         int b) {
+__device__ __forceinline__ void uf_unite(int32_t *parent, int a, int b) {
     while (true) {
         a = uf_find(parent, a);
         b = uf_find(parent, b);
         if (a == b) return;
-        uint32_t sa = (uint32_t)skey[a] & smask, sb = (uint32_t)skey[b] & smask;
-        if (sa < sb) {
-            int t = a;
+        if (a > b) {
+            const int t = a;
             a = b;
             b = t;
         }
-        int old = atomicCAS(parent + a, a, b);
-        if (old == a) return;
+#ifdef RTDBS_TIMING_PLAIN_HOOK
+        parent[b] = a;  // racy: timing experiments only
+        return;
+#endif
+        const int old = atomicMin(parent + b, a);
+        if (old == b) return;
+        b = old;
     }
+}
+
+__device__ __forceinline__ int slot_of(const StreamDev &S, int pos) {
+    return (int)((uint32_t)S.skey[pos] & (uint32_t)((1ull << S.slot_bits) - 1ull));
 }
 
 // ---------------------------------------------------------------------------
@@ -213,6 +241,34 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     x *= 0xc4ceb9fe1a85ec53ULL;
     x ^= x >> 33;
     return x;
+}
+
+// Quotient and remainder of a cell key (< 2^40) by a grid extent (< 2^31),
+// via a double-precision estimate and an exact integer fix-up (64-bit integer
+// division is a long software sequence on the GPU).
+__device__ __forceinline__ void divmod_key(uint64_t x, int64_t d, int64_t *q, int64_t *r) {
+    int64_t qq = (int64_t)((double)x / (double)d);
+    int64_t rr = (int64_t)x - qq * d;
+    while (rr < 0) {
+        qq--;
+        rr += d;
+    }
+    while (rr >= d) {
+        qq++;
+        rr -= d;
+    }
+    *q = qq;
+    *r = rr;
+}
+
+__device__ __forceinline__ void decode_key(const StreamDev &S, uint64_t ck, int64_t ix[3]) {
+    int64_t q, r;
+    divmod_key(ck, S.n[2], &q, &r);
+    ix[2] = r;
+    int64_t q2, r2;
+    divmod_key((uint64_t)q, S.n[1], &q2, &r2);
+    ix[1] = r2;
+    ix[0] = q2;
 }
 
 // Open-addressing cell hash; an entry whose epoch differs from the current
@@ -250,16 +306,24 @@ __device__ __forceinline__ int hash_find(const StreamDev &S, uint64_t ck, uint32
 // sorting: LDS bitonic tile sort + merge-path merges
 
 __device__ __forceinline__ void bitonic_sort_tile(uint64_t *sk) {
+    constexpr int kPer = kTile / 2 / kTileThreads;  // compare-exchanges per thread and stage
     for (int k = 2; k <= kTile; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int q = threadIdx.x; q < kTile / 2; q += kTileThreads) {
-                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                const int ixj = i + j;
-                const uint64_t a = sk[i], b = sk[ixj];
-                const bool up = (i & k) == 0;
-                if ((a > b) == up) {
-                    sk[i] = b;
-                    sk[ixj] = a;
+            int ia[kPer];
+            uint64_t va[kPer], vb[kPer];
+#pragma unroll
+            for (int t = 0; t < kPer; t++) {
+                const int q = threadIdx.x + t * kTileThreads;
+                ia[t] = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                va[t] = sk[ia[t]];
+                vb[t] = sk[ia[t] + j];
+            }
+#pragma unroll
+            for (int t = 0; t < kPer; t++) {
+                const bool up = (ia[t] & k) == 0;
+                if ((va[t] > vb[t]) == up) {
+                    sk[ia[t]] = vb[t];
+                    sk[ia[t] + j] = va[t];
                 }
             }
             __syncthreads();
@@ -309,12 +373,25 @@ __global__ __launch_bounds__(kTileThreads) void k_stride_tiles(BatchArgs a) {
 }
 
 // Number of elements of A among the first d of merge(A, B); keys are unique.
-__device__ __forceinline__ int merge_path(const uint64_t *A, int la, const uint64_t *B, int lb, int d) {
+// P(i) = A[i] < B[d-1-i] holds exactly for i below the answer. One wave
+// samples P at 64 evenly spaced points per round, so the search costs
+// ceil(log64(range)) dependent round trips instead of log2(range).
+__device__ __forceinline__ int merge_path_wave(const uint64_t *A, int la, const uint64_t *B, int lb, int d) {
+    const int lane = lane_id();
     int lo = max(0, d - lb), hi = min(d, la);
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (A[mid] < B[d - 1 - mid]) lo = mid + 1;
-        else hi = mid;
+    while (hi > lo) {
+        const int n = hi - lo;
+        const int step = (n + 63) >> 6;
+        const int i = lo + lane * step;
+        bool p = false;
+        if (i < hi) p = A[i] < B[d - 1 - i];
+        const int cnt = __popcll(__ballot(p));
+        if (step == 1) return lo + cnt;
+        if (cnt == 0) return lo;
+        const int nlo = lo + (cnt - 1) * step + 1;
+        const int nhi = min(hi, lo + cnt * step);
+        lo = nlo;
+        hi = nhi;
     }
     return lo;
 }
@@ -331,7 +408,11 @@ struct MergeLds {
 __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint64_t *B, int lb, int d0,
                                            int d1, MergeLds &m, int *a0_out, int *b0_out) {
     const int tid = threadIdx.x;
-    if (tid < 2) m.sp[tid] = merge_path(A, la, B, lb, tid ? d1 : d0);
+    if (tid < 128) {
+        const int sp = merge_path_wave(A, la, B, lb, tid < 64 ? d0 : d1);
+        if (tid == 0) m.sp[0] = sp;
+        if (tid == 64) m.sp[1] = sp;
+    }
     __syncthreads();
     const int a0 = m.sp[0], a1 = m.sp[1];
     const int b0 = d0 - a0, b1 = d1 - a1;
@@ -531,6 +612,8 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a, int b
 // so no sort and no host round trip (dbscan.cpp:549) are needed.
 __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     __shared__ int lds4[4];
+    __shared__ int newc[kTile];        // this tile's new cells ...
+    __shared__ uint64_t newk[kTile];   // ... and their keys
     const int g = blockIdx.x;
     const int s = find_stream(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
@@ -559,9 +642,15 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
             const int c = base + run + rank;
             S.cstart[c] = p;
             S.ckey[c] = ck;
-            hash_insert(S, ck, c, epoch);
+            newc[run + rank] = c;
+            newk[run + rank] = ck;
         }
         run += tot;
+    }
+    __syncthreads();
+    // hash inserts after the scan, spread over the block
+    for (int i = threadIdx.x; i < run; i += kTileThreads) {
+        hash_insert(S, newk[i], newc[i], epoch);
     }
     if (tile == ntile - 1 && threadIdx.x == 0) {
         const int total = base + run;
@@ -576,12 +665,8 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
 // only the fastest axis, so its cells are consecutive in the sorted table.
 
 __device__ __forceinline__ int2 cell_row(const StreamDev &S, int c, int r, uint32_t epoch) {
-    const uint64_t ck = S.ckey[c];
     int64_t ix[3];
-    ix[2] = (int64_t)(ck % (uint64_t)S.n[2]);
-    const uint64_t rem = ck / (uint64_t)S.n[2];
-    ix[1] = (int64_t)(rem % (uint64_t)S.n[1]);
-    ix[0] = (int64_t)(rem / (uint64_t)S.n[1]);
+    decode_key(S, S.ckey[c], ix);
     const int k = S.k, last = S.dim - 1, w = 2 * k + 1;
     const int center = (S.R - 1) / 2;
     int rr = r == 0 ? center : (r == center ? 0 : r);  // centre row first
@@ -594,45 +679,77 @@ __device__ __forceinline__ int2 cell_row(const StreamDev &S, int c, int r, uint3
     }
     const int64_t zlo = ix[last] - k < 0 ? 0 : ix[last] - k;
     const int64_t zhi = ix[last] + k >= S.n[last] ? S.n[last] - 1 : ix[last] + k;
+    const int nz = (int)(zhi - zlo + 1);
+    // first probe of every cell of the row, issued together
+    uint64_t tag[2 * kMaxK + 1];
+    uint32_t h[2 * kMaxK + 1];
+    uint64_t cur[2 * kMaxK + 1];
+#pragma unroll
+    for (int i = 0; i < 2 * kMaxK + 1; i++) {
+        if (i < nz) {
+            jx[last] = zlo + i;
+            const uint64_t key = (uint64_t)((jx[0] * S.n[1] + jx[1]) * S.n[2] + jx[2]);
+            tag[i] = ((uint64_t)epoch << kKeyBits) | key;
+            h[i] = (uint32_t)mix64(key) & S.hmask;
+            cur[i] = S.htab[h[i]];
+        }
+    }
     int first = -1, lastc = -1;
-    for (int64_t z = zlo; z <= zhi; z++) {
-        jx[last] = z;
-        const uint64_t key = (uint64_t)((jx[0] * S.n[1] + jx[1]) * S.n[2] + jx[2]);
-        const int c2 = hash_find(S, key, epoch);
-        if (c2 >= 0) {
-            if (first < 0) first = c2;
-            lastc = c2;
+#pragma unroll
+    for (int i = 0; i < 2 * kMaxK + 1; i++) {
+        if (i < nz) {
+            // continue the probe sequence where the first probe was inconclusive
+            uint32_t hh = h[i];
+            uint64_t cv = cur[i];
+            while (cv != tag[i] && (uint32_t)(cv >> kKeyBits) == epoch) {
+                hh = (hh + 1) & S.hmask;
+                cv = S.htab[hh];
+            }
+            if (cv == tag[i]) {
+                const int c2 = S.hval[hh];
+                if (first < 0) first = c2;
+                lastc = c2;
+            }
         }
     }
     if (first < 0) return make_int2(0, 0);
     return make_int2(S.cstart[first], S.cstart[lastc + 1]);
 }
 
-// A cell whose >= min_pts points are all core without counting: the grid is
-// dense-safe (k*k > dim: any two points of one cell are closer than eps) and
-// the cell is interior. Border cells may hold points clamped in from outside
-// the grid bounds, which the side-length argument does not cover.
-__device__ __forceinline__ bool cell_is_dense(const StreamDev &S, int c) {
-    const int cnt = S.cstart[c + 1] - S.cstart[c];
-    if (!S.dense_ok || cnt < S.min_pts) return false;
-    const uint64_t ck = S.ckey[c];
+// Cell classes (written once per slide by k_rows):
+//   kCellSafe  - the grid is dense-safe (k*k > dim: any two points of one
+//                cell are closer than eps) and the cell is interior; border
+//                cells may hold points clamped in from outside the grid
+//                bounds, which the side-length argument does not cover.
+//   kCellDense - safe and holding >= min_pts points: all of them are core.
+__device__ __forceinline__ uint8_t cell_class(const StreamDev &S, int c) {
+    if (!S.dense_ok) return 0;
     int64_t ix[3];
-    ix[2] = (int64_t)(ck % (uint64_t)S.n[2]);
-    const uint64_t rem = ck / (uint64_t)S.n[2];
-    ix[1] = (int64_t)(rem % (uint64_t)S.n[1]);
-    ix[0] = (int64_t)(rem / (uint64_t)S.n[1]);
+    decode_key(S, S.ckey[c], ix);
     for (int a = 0; a < S.dim; a++)
-        if (ix[a] <= 0 || ix[a] >= S.n[a] - 1) return false;
-    return true;
+        if (ix[a] <= 0 || ix[a] >= S.n[a] - 1) return 0;
+    const int cnt = S.cstart[c + 1] - S.cstart[c];
+    return cnt >= S.min_pts ? (kCellSafe | kCellDense) : kCellSafe;
 }
 
 // Persistent: walks the device-side list of (stream, cell, row) items. The
-// row-0 item of each cell also records whether the cell is dense.
+// row-0 item of each cell also records the cell's class; block 0 publishes
+// the per-stream prefix of cell counts for the cell-parallel kernels.
 __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
     __shared__ long long pre[kMaxBatch + 1];
     __shared__ long long tmp4[4];
     block_stream_prefix(a.n, pre, tmp4,
                         [&](int i) { return (long long)a.s[i].mut->ncell * a.s[i].R; });
+    if (blockIdx.x == 0) {
+        long long run = 0;
+        if (threadIdx.x == 0) {
+            a.cell_pre[0] = 0;
+            for (int i = 0; i < a.n; i++) {
+                run += a.s[i].mut->ncell;
+                a.cell_pre[i + 1] = (int32_t)run;
+            }
+        }
+    }
     const long long total = pre[a.n];
     for (long long gi = (long long)blockIdx.x * kTileThreads + threadIdx.x; gi < total;
          gi += (long long)gridDim.x * kTileThreads) {
@@ -641,53 +758,81 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
         const long long local = gi - pre[s];
         const int c = (int)(local / S.R), r = (int)(local % S.R);
         S.rows[(size_t)c * S.R + r] = cell_row(S, c, r, S.mut->epoch);
-        if (r == 0) S.cdense[c] = cell_is_dense(S, c) ? 1 : 0;
+        if (r == 0) {
+            S.cflag[c] = cell_class(S, c);
+            S.fcore[c] = kNoCore;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// neighbourhood iteration: a 64-thread block (one wave) stages the stencil
-// candidates of one cell into LDS, kCap at a time, and calls body(fill) when
-// at least flush_at candidates are staged (and at the end). body returns
-// false to stop early.
+// cell-parallel neighbourhood kernels: one wave per (stream, cell), walking
+// the device-side cell list in a persistent loop. The candidates of a cell
+// are the concatenation of its stencil rows; lanes 0..R-1 hold one row each.
 
-template <class Stage, class Body>
-__device__ __forceinline__ void for_candidates(const StreamDev &S, int c, int flush_at, Stage stage,
-                                               Body body) {
-    const int lane = lane_id();
-    int2 myrow = make_int2(0, 0);
-    if (lane < S.R) myrow = S.rows[(size_t)c * S.R + lane];
-    int fill = 0;
-    for (int r = 0; r < S.R; r++) {
-        int rb = __shfl(myrow.x, r);
-        const int re = __shfl(myrow.y, r);
-        while (rb < re) {
-            const int take = min(re - rb, kCap - fill);
-            for (int j = lane; j < take; j += 64) stage(fill + j, rb + j);
-            fill += take;
-            rb += take;
-            if (fill == kCap) {
-                __syncthreads();
-                const bool more = body(fill);
-                __syncthreads();
-                fill = 0;
-                if (!more) return;
-            }
-        }
-        if (fill >= flush_at) {
-            __syncthreads();
-            const bool more = body(fill);
-            __syncthreads();
-            fill = 0;
-            if (!more) return;
-        }
-    }
-    if (fill > 0) {
-        __syncthreads();
-        body(fill);
-        __syncthreads();
-    }
+struct RowScan {
+    int rb;     // lane r: first window position of row r
+    int pre;    // lane r: candidates in rows before r
+    int total;  // uniform: candidates of the cell
+};
+
+__device__ __forceinline__ int2 load_row(const StreamDev &S, int c) {
+    int2 r = make_int2(0, 0);
+    if (lane_id() < S.R) r = S.rows[(size_t)c * S.R + lane_id()];
+    return r;
 }
+
+__device__ __forceinline__ RowScan row_scan_of(int2 r) {
+    const int lane = lane_id();
+    const int len = r.y - r.x;
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    RowScan rs;
+    rs.rb = r.x;
+    rs.pre = incl - len;
+    rs.total = __shfl(incl, 63);
+    return rs;
+}
+
+__device__ __forceinline__ RowScan row_scan(const StreamDev &S, int c) { return row_scan_of(load_row(S, c)); }
+
+// Index r of the row holding candidate j (largest r < R with pre(r) <= j).
+// Every lane takes part; j may differ per lane.
+__device__ __forceinline__ int row_of(int pre, int R, int j) {
+    int lo = 0, hi = R;
+#pragma unroll
+    for (int it = 0; it < 6; it++) {
+        const int mid = (lo + hi) >> 1;
+        const int pm = __shfl(pre, mid);
+        if (hi - lo > 1) {
+            if (pm <= j) lo = mid;
+            else hi = mid;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int cand_pos(const RowScan &rs, int R, int j) {
+    const int r = row_of(rs.pre, R, j);
+    return __shfl(rs.rb, r) + (j - __shfl(rs.pre, r));
+}
+
+struct CellCtx {
+    int s, c;
+};
+
+__device__ __forceinline__ CellCtx cell_of(const BatchArgs &a, int gc) {
+    CellCtx x;
+    x.s = find_stream(a.cell_pre, a.n, gc);
+    x.c = gc - a.cell_pre[x.s];
+    return x;
+}
+
+constexpr int kBatchM = 4;  // candidates held in registers per lane and pass
 
 struct ChunkCtx {
     int s, p0, p1;
@@ -702,297 +847,504 @@ __device__ __forceinline__ ChunkCtx chunk_of(const BatchArgs &a, int g) {
     return x;
 }
 
-__device__ __forceinline__ bool is_dense(const StreamDev &S, int c) { return S.cdense[c] != 0; }
+// The union-find element (a slot) representing a core: the slot of the first
+// core of its cell when the cell is safe (all its points are within eps of
+// each other, so all its cores belong to one cluster), else its own slot.
+__device__ __forceinline__ int rep_slot(const StreamDev &S, int j) {
+    if (!S.dense_ok) return slot_of(S, j);
+    const int cj = S.pcell[j];
+    return slot_of(S, (S.cflag[cj] & kCellSafe) ? S.fcore[cj] : j);
+}
 
 // Range count = core identification (the reference's __raygen__grid +
 // __intersection__grid, dbscan.cu:155-194,250-270, and set_label,
-// aabb.cu:575-587). Per cell segment of the chunk, candidates are staged in
-// LDS and each query's neighbours are counted 64 at a time with
-// ballot/popcount, stopping at min_pts. Cells of a dense-safe grid holding
-// >= min_pts points are core without counting (all pairs within eps).
+// aabb.cu:575-587), one wave per cell. Dense cells are core without counting
+// (all pairs within eps). Otherwise each lane holds up to kBatchM stencil
+// candidates in registers and every query of the cell is tested against all
+// of them at once, neighbours counted with ballot/popcount up to min_pts.
+// The cell's first core becomes its union-find representative; cells of
+// >= kBoxMin points also record their bounding box.
 __global__ __launch_bounds__(64) void k_count(BatchArgs a) {
-    __shared__ float cx[kCap], cy[kCap], cz[kCap];
-    const ChunkCtx x = chunk_of(a, blockIdx.x);
-    const StreamDev &S = a.s[x.s];
     const int lane = lane_id();
-    const int p = x.p0 + lane;
-    const bool mine = p < x.p1;
-    const int dim = S.dim, minpts = S.min_pts;
-    const float e2 = S.eps2;
-    const int cfirst = S.pcell[x.p0], clast = S.pcell[x.p1 - 1];
-    int mycnt = 0;
-    for (int c = cfirst; c <= clast; c++) {
+    const int total = a.cell_pre[a.n];
+    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
+        const CellCtx x = cell_of(a, gc);
+        const StreamDev &S = a.s[x.s];
+        const int c = x.c;
         const int cs = S.cstart[c], ce = S.cstart[c + 1];
-        const int q0 = max(cs, x.p0), q1 = min(ce, x.p1);
-        const bool inseg = mine && p >= q0 && p < q1;
-        if (is_dense(S, c)) {
-            if (inseg) {
-                S.flags[p] = kFlagCore | kFlagDense;
-                S.parent[p] = cs;  // first position of the cell holds its minimum slot
+        const uint8_t cf = S.cflag[c];
+        const int nq = ce - cs;
+        const int2 myrow = load_row(S, c);
+        const int dim = S.dim, minpts = S.min_pts;
+        const float e2 = S.eps2;
+        // lane q holds query q of the current block of 64
+        float mqx = 0.f, mqy = 0.f, mqz = 0.f;
+        if (lane < nq) {
+            mqx = S.sx[cs + lane];
+            if (dim > 1) mqy = S.sy[cs + lane];
+            if (dim > 2) mqz = S.sz[cs + lane];
+        }
+        if (nq >= kBoxMin) {
+            float mn[3], mx[3];
+            const float v0[3] = {mqx, mqy, mqz};
+            for (int d = 0; d < 3; d++) {
+                mn[d] = lane < nq ? v0[d] : INFINITY;
+                mx[d] = lane < nq ? v0[d] : -INFINITY;
             }
+            for (int i = cs + 64 + lane; i < ce; i += 64) {
+                const float v[3] = {S.sx[i], dim > 1 ? S.sy[i] : 0.f, dim > 2 ? S.sz[i] : 0.f};
+                for (int d = 0; d < 3; d++) {
+                    mn[d] = fminf(mn[d], v[d]);
+                    mx[d] = fmaxf(mx[d], v[d]);
+                }
+            }
+            for (int d = 0; d < 3; d++) {
+                for (int o = 32; o > 0; o >>= 1) {
+                    mn[d] = fminf(mn[d], __shfl_xor(mn[d], o));
+                    mx[d] = fmaxf(mx[d], __shfl_xor(mx[d], o));
+                }
+            }
+            if (lane < 6) {
+                float v = mn[0];
+                if (lane == 1) v = mn[1];
+                if (lane == 2) v = mn[2];
+                if (lane == 3) v = mx[0];
+                if (lane == 4) v = mx[1];
+                if (lane == 5) v = mx[2];
+                S.cbox[(size_t)c * 6 + lane] = v;
+            }
+        }
+        if (cf & kCellDense) {
+            for (int i = lane; i < nq; i += 64) {
+                S.flags[cs + i] = kFlagCore | kFlagDense;
+                const int sl = slot_of(S, cs + i);
+                S.parent[sl] = sl;
+            }
+            if (lane == 0) S.fcore[c] = cs;  // first position of the cell holds its minimum slot
             continue;
         }
-        mycnt = 0;
-        for_candidates(
-            S, c, 128,
-            [&](int i, int pos) {
-                cx[i] = S.sx[pos];
-                if (dim > 1) cy[i] = S.sy[pos];
-                if (dim > 2) cz[i] = S.sz[pos];
-            },
-            [&](int fill) {
-                for (int jl = q0 - x.p0; jl < q1 - x.p0; jl++) {
-                    int cj = __shfl(mycnt, jl);
-                    if (cj >= minpts) continue;
-                    const int pq = x.p0 + jl;
-                    const float qx = S.sx[pq];
-                    const float qy = dim > 1 ? S.sy[pq] : 0.f;
-                    const float qz = dim > 2 ? S.sz[pq] : 0.f;
-                    for (int i = 0; i < fill && cj < minpts; i += 64) {
-                        const int idx = i + lane;
-                        bool hit = false;
-                        if (idx < fill) hit = sqd(dim, qx, qy, qz, cx[idx], cy[idx], cz[idx]) < e2;
-                        cj += __popcll(__ballot(hit));
-                    }
-                    if (lane == jl) mycnt = cj;
+        const RowScan rs = row_scan_of(myrow);
+        const int T = rs.total, R = S.R;
+        int fc = kNoCore;
+        for (int qb = 0; qb < nq; qb += 64) {
+            const int qn = min(64, nq - qb);
+            if (qb > 0) {
+                mqx = mqy = mqz = 0.f;
+                if (lane < qn) {
+                    mqx = S.sx[cs + qb + lane];
+                    if (dim > 1) mqy = S.sy[cs + qb + lane];
+                    if (dim > 2) mqz = S.sz[cs + qb + lane];
                 }
-                return __ballot(inseg && mycnt < minpts) != 0ull;
-            });
-        if (inseg) {
-            S.flags[p] = mycnt >= minpts ? kFlagCore : 0;
-            S.parent[p] = p;
+            }
+            int mycnt = 0;
+            for (int cb = 0; cb < T; cb += 64 * kBatchM) {
+                float px[kBatchM], py[kBatchM], pz[kBatchM];
+                bool v[kBatchM];
+#pragma unroll
+                for (int m = 0; m < kBatchM; m++) {
+                    const int j = cb + 64 * m + lane;
+                    v[m] = j < T;
+                    px[m] = py[m] = pz[m] = 0.f;
+                    if (cb + 64 * m < T) {
+                        const int pos = cand_pos(rs, R, v[m] ? j : T - 1);
+                        px[m] = S.sx[pos];
+                        if (dim > 1) py[m] = S.sy[pos];
+                        if (dim > 2) pz[m] = S.sz[pos];
+                    }
+                }
+                for (int q = 0; q < qn; q++) {
+                    int cj = lane_i(mycnt, q);
+                    if (cj >= minpts) continue;
+                    const float qx = lane_f(mqx, q), qy = lane_f(mqy, q), qz = lane_f(mqz, q);
+#pragma unroll
+                    for (int m = 0; m < kBatchM; m++) {
+                        if (cb + 64 * m < T) {
+                            const bool hit = v[m] && sqd(dim, qx, qy, qz, px[m], py[m], pz[m]) < e2;
+                            cj += __popcll(__ballot(hit));
+                        }
+                    }
+                    if (lane == q) mycnt = cj;
+                }
+                if (__ballot(lane < qn && mycnt < minpts) == 0ull) break;
+            }
+            const bool core = lane < qn && mycnt >= minpts;
+            if (fc == kNoCore) {
+                const uint64_t m = __ballot(core);
+                if (m) fc = cs + qb + __ffsll((unsigned long long)m) - 1;
+            }
+            if (lane < qn) {
+                const int p = cs + qb + lane;
+                S.flags[p] = core ? kFlagCore : 0;
+                const int sl = slot_of(S, p);
+                S.parent[sl] = sl;
+            }
         }
+        if (lane == 0) S.fcore[c] = fc;
     }
 }
 
-// Cluster merge, point level (the reference's __raygen__cluster +
-// __intersection__hybrid_radius_sphere core branch, dbscan.cu:279-321,
-// 354-394): every core of a sparse cell unites with every core within eps
-// that lies in a dense cell or at a smaller position (each sparse-sparse pair
-// once). Pairs inside a dense cell are pre-linked; dense-dense pairs are
-// handled per cell pair by k_union_dense.
-__global__ __launch_bounds__(64) void k_union_sparse(BatchArgs a) {
-    __shared__ float cx[kCap], cy[kCap], cz[kCap];
-    __shared__ int32_t cpos[kCap];
-    __shared__ uint8_t cfl[kCap];
-    const ChunkCtx x = chunk_of(a, blockIdx.x);
-    const StreamDev &S = a.s[x.s];
-    const int lane = lane_id();
-    const int p = x.p0 + lane;
-    const bool mine = p < x.p1;
+// Conservative bounding-box tests between two cells (double arithmetic): a
+// pair passing the FP32 test has true distance below eps*(1 + 2^-21).
+__device__ __forceinline__ int box_relation(const StreamDev &S, int A, int B) {
+    double mn2 = 0.0, mx2 = 0.0;
+    for (int d = 0; d < S.dim; d++) {
+        const double alo = S.cbox[(size_t)A * 6 + d], ahi = S.cbox[(size_t)A * 6 + 3 + d];
+        const double blo = S.cbox[(size_t)B * 6 + d], bhi = S.cbox[(size_t)B * 6 + 3 + d];
+        const double gap = fmax(blo - ahi, alo - bhi);
+        if (gap > 0.0) mn2 += gap * gap;
+        const double span = fmax(bhi - alo, ahi - blo);
+        mx2 += span * span;
+    }
+    const double e2 = (double)S.eps2;
+    if (mn2 >= e2 * (1.0 + 1.0 / 1048576.0)) return -1;  // no pair within eps
+    if (mx2 * (1.0 + 1.0 / 1048576.0) < e2) return 1;    // every pair within eps
+    return 0;
+}
+
+// True if some core of cell A and some core of cell B are within eps.
+__device__ __forceinline__ bool cells_touch(const StreamDev &S, int A, int B, uint8_t cfA, uint8_t cfB) {
+    const int csA = S.cstart[A], ceA = S.cstart[A + 1];
+    const int csB = S.cstart[B], ceB = S.cstart[B + 1];
+    if (ceA - csA >= kBoxMin && ceB - csB >= kBoxMin) {
+        const int rel = box_relation(S, A, B);
+        if (rel < 0) return false;
+        if (rel > 0) return true;  // both cells hold a core
+    }
     const int dim = S.dim;
     const float e2 = S.eps2;
-    const uint32_t sm = slot_mask(S);
-    const bool mycore = mine && (S.flags[p] & kFlagCore);
-    const int cfirst = S.pcell[x.p0], clast = S.pcell[x.p1 - 1];
-    for (int c = cfirst; c <= clast; c++) {
-        const int cs = S.cstart[c], ce = S.cstart[c + 1];
-        if (is_dense(S, c)) continue;
-        const int q0 = max(cs, x.p0), q1 = min(ce, x.p1);
-        const bool qcore = mycore && p >= q0 && p < q1;
-        if (__ballot(qcore) == 0ull) continue;
-        for_candidates(
-            S, c, kCap,
-            [&](int i, int pos) {
-                cx[i] = S.sx[pos];
-                if (dim > 1) cy[i] = S.sy[pos];
-                if (dim > 2) cz[i] = S.sz[pos];
-                cpos[i] = pos;
-                cfl[i] = S.flags[pos];
-            },
-            [&](int fill) {
-                const uint64_t qm = __ballot(qcore);
-                for (int jl = q0 - x.p0; jl < q1 - x.p0; jl++) {
-                    if (!((qm >> jl) & 1ull)) continue;
-                    const int pq = x.p0 + jl;
-                    const float qx = S.sx[pq];
-                    const float qy = dim > 1 ? S.sy[pq] : 0.f;
-                    const float qz = dim > 2 ? S.sz[pq] : 0.f;
-                    for (int i = 0; i < fill; i += 64) {
-                        const int idx = i + lane;
-                        if (idx < fill) {
-                            const uint8_t f = cfl[idx];
-                            const int cp = cpos[idx];
-                            if ((f & kFlagCore) && ((f & kFlagDense) || cp < pq) &&
-                                sqd(dim, qx, qy, qz, cx[idx], cy[idx], cz[idx]) < e2)
-                                uf_unite(S.parent, S.skey, sm, pq, cp);
-                        }
-                    }
-                }
+    const bool allA = cfA & kCellDense, allB = cfB & kCellDense;
+    for (int i = csA; i < ceA; i++) {
+        if (!allA && !(S.flags[i] & kFlagCore)) continue;
+        const float ax = S.sx[i], ay = dim > 1 ? S.sy[i] : 0.f, az = dim > 2 ? S.sz[i] : 0.f;
+        for (int j = csB; j < ceB; j++) {
+            if (!allB && !(S.flags[j] & kFlagCore)) continue;
+            if (sqd(dim, ax, ay, az, S.sx[j], dim > 1 ? S.sy[j] : 0.f, dim > 2 ? S.sz[j] : 0.f) < e2)
                 return true;
-            });
+        }
     }
+    return false;
 }
 
-// Cluster merge, cell level: two dense cells (all core, pre-linked) join as
-// soon as one pair of their points is within eps - the analogue of the
+// Per-wave edge buffer in LDS: touching pairs are collected while the wave
+// walks its cells and united lane-parallel when the buffer fills (and at the
+// end), so the latency of the union-find walks overlaps across cells instead
+// of adding up cell after cell. Appending needs no atomics.
+constexpr int kEdgeBuf = 512;
+
+__device__ __forceinline__ void flush_edges(const StreamDev &S, int2 *ebuf, int &ecnt) {
+#ifndef RTDBS_TIMING_NO_UNITE
+    for (int i = lane_id(); i < ecnt; i += 64) uf_unite(S.parent, ebuf[i].x, ebuf[i].y);
+#endif
+    ecnt = 0;
+}
+
+__device__ __forceinline__ void unite_if(const StreamDev &S, int2 *ebuf, int &ecnt, bool want, int u, int v) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return;
+    const int n = __popcll(m);
+    if (ecnt + n > kEdgeBuf) flush_edges(S, ebuf, ecnt);
+    if (want) ebuf[ecnt + __popcll(m & ((1ull << lane_id()) - 1ull))] = make_int2(u, v);
+    ecnt += n;
+}
+
+// Cluster merge, cell level, for safe cells: all cores of a safe cell belong
+// to one cluster (its points are within eps of each other), so two safe cells
+// join as soon as one core pair is within eps - the analogue of the
 // reference's dense-cell spheres, where a ray unites with the first point of
-// the cell within eps and stops (dbscan.cu:378-393). Persistent over the
-// device-side cell lists; one wave per cell, each unordered pair once.
-__global__ __launch_bounds__(64) void k_union_dense(BatchArgs a) {
-    __shared__ long long pre[kMaxBatch + 1];
-    __shared__ long long tmp4[4];
-    // block_stream_prefix assumes 256 threads; emulate with a serial prefix
-    // over at most kMaxBatch streams (one wave, 16 per lane).
-    {
-        const int lane = lane_id();
-        long long local = 0;
-        long long v[kMaxBatch / 64];
-        for (int q = 0; q < kMaxBatch / 64; q++) {
-            const int i = lane * (kMaxBatch / 64) + q;
-            v[q] = i < a.n ? (long long)a.s[i].mut->ncell : 0;
-            local += v[q];
-        }
-        long long xs = local;
-        for (int o = 1; o < 64; o <<= 1) {
-            long long y = __shfl_up(xs, o);
-            if (lane >= o) xs += y;
-        }
-        long long run = xs - local;
-        for (int q = 0; q < kMaxBatch / 64; q++) {
-            const int i = lane * (kMaxBatch / 64) + q;
-            run += v[q];
-            if (i < a.n) pre[i + 1] = run;
-        }
-        if (lane == 0) pre[0] = 0;
-        (void)tmp4;
-        __syncthreads();
-    }
-    const long long total = pre[a.n];
+// the cell within eps and stops (dbscan.cu:378-393). Each lane takes one
+// stencil cell B < A, so each unordered pair is tested once, and unites the
+// two cells' representatives when they touch. A's points sit in LDS; a small
+// B is held in the lane's registers, so the pair test is arithmetic only.
+constexpr int kUnionA = 256;  // LDS capacity for the query cell
+constexpr int kUnionB = 8;    // register capacity for a candidate cell
+
+__global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
+    __shared__ float lx[kUnionA], ly[kUnionA], lz[kUnionA];
+    __shared__ uint8_t lc[kUnionA];
+    __shared__ int2 ebuf[kEdgeBuf];
     const int lane = lane_id();
-    for (long long gc = blockIdx.x; gc < total; gc += gridDim.x) {
-        const int s = find_stream(pre, a.n, gc);
-        const StreamDev &S = a.s[s];
-        const int c = (int)(gc - pre[s]);
-        const int cs = S.cstart[c], ce = S.cstart[c + 1];
-        if (!is_dense(S, c)) continue;
+    const int total = a.cell_pre[a.n];
+    int ecnt = 0;
+    int es = -1;  // stream of the buffered edges
+    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
+        const CellCtx x = cell_of(a, gc);
+        const StreamDev &S = a.s[x.s];
+        if (x.s != es) {
+            if (es >= 0) {
+                __syncthreads();
+                flush_edges(a.s[es], ebuf, ecnt);
+            }
+            es = x.s;
+        }
+        const int A = x.c;
+        const uint8_t cfA = S.cflag[A];
+        if (!(cfA & kCellSafe)) continue;
+        const int fcA = S.fcore[A];
+        if (fcA == kNoCore) continue;
         const int dim = S.dim;
         const float e2 = S.eps2;
-        const uint32_t sm = slot_mask(S);
-        int2 myrow = make_int2(0, 0);
-        if (lane < S.R) myrow = S.rows[(size_t)c * S.R + lane];
-        for (int r = 0; r < S.R; r++) {
-            const int rb = __shfl(myrow.x, r), re = __shfl(myrow.y, r);
-            if (rb >= re) continue;
-            const int c2lo = S.pcell[rb];
-            const int c2hi = min(S.pcell[re - 1], c - 1);
-            for (int c2 = c2lo; c2 <= c2hi; c2++) {
-                const int cs2 = S.cstart[c2], ce2 = S.cstart[c2 + 1];
-                if (!is_dense(S, c2)) continue;
-                int same = 0;
-                if (lane == 0) same = uf_find(S.parent, cs) == uf_find(S.parent, cs2);
-                if (__shfl(same, 0)) continue;
-                bool found = false;
-                for (int b0 = cs2; b0 < ce2 && !found; b0 += 64) {
-                    const int pb = b0 + lane;
-                    const bool valid = pb < ce2;
-                    float bx = 0.f, by = 0.f, bz = 0.f;
-                    if (valid) {
-                        bx = S.sx[pb];
-                        if (dim > 1) by = S.sy[pb];
-                        if (dim > 2) bz = S.sz[pb];
+        // stencil cells B < A, row by row: [pcell[rb], min(pcell[re-1], A-1)]
+        int clo = 0, cnt = 0;
+        if (lane < S.R) {
+            const int2 r = S.rows[(size_t)A * S.R + lane];
+            if (r.x < r.y) {
+                clo = S.pcell[r.x];
+                const int chi = min(S.pcell[r.y - 1], A - 1);
+                cnt = max(0, chi - clo + 1);
+            }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const int pre = incl - cnt;
+        const int TC = __shfl(incl, 63);
+        if (TC == 0) continue;
+        const int csA = S.cstart[A], ceA = S.cstart[A + 1], nA = ceA - csA;
+        const bool inl = nA <= kUnionA;
+        const int sA = slot_of(S, fcA);
+        __syncthreads();  // the previous cell's readers are done with the LDS
+        if (inl) {
+            for (int i = lane; i < nA; i += 64) {
+                lx[i] = S.sx[csA + i];
+                ly[i] = dim > 1 ? S.sy[csA + i] : 0.f;
+                lz[i] = dim > 2 ? S.sz[csA + i] : 0.f;
+                lc[i] = (cfA & kCellDense) ? (uint8_t)1 : (uint8_t)(S.flags[csA + i] & kFlagCore);
+            }
+        }
+        __syncthreads();
+        for (int jb = 0; jb < TC; jb += 64) {
+            const int j = jb + lane;
+            const int jj = j < TC ? j : TC - 1;
+            const int r = row_of(pre, S.R, jj);
+            const int B = __shfl(clo, r) + (jj - __shfl(pre, r));
+            bool big = false;
+            bool touch = false;
+            int fcB = kNoCore;
+            if (j < TC) {
+                const uint8_t cfB = S.cflag[B];
+                fcB = S.fcore[B];
+                if ((cfB & kCellSafe) && fcB != kNoCore) {
+                    const int csB = S.cstart[B], nB = S.cstart[B + 1] - csB;
+                    if (inl && nB <= kUnionB) {
+                        // small cell: its points in registers, A's cores from LDS
+                        float bx[kUnionB], by[kUnionB], bz[kUnionB];
+                        bool bc[kUnionB];
+#pragma unroll
+                        for (int m = 0; m < kUnionB; m++) {
+                            bc[m] = false;
+                            bx[m] = by[m] = bz[m] = 0.f;
+                            if (m < nB) {
+                                bc[m] = (cfB & kCellDense) || (S.flags[csB + m] & kFlagCore);
+                                bx[m] = S.sx[csB + m];
+                                if (dim > 1) by[m] = S.sy[csB + m];
+                                if (dim > 2) bz[m] = S.sz[csB + m];
+                            }
+                        }
+                        for (int i = 0; i < nA && !touch; i++) {
+                            if (!lc[i]) continue;
+                            const float qx = lx[i], qy = ly[i], qz = lz[i];
+#pragma unroll
+                            for (int m = 0; m < kUnionB; m++)
+                                touch = touch || (bc[m] && sqd(dim, qx, qy, qz, bx[m], by[m], bz[m]) < e2);
+                        }
+                    } else {
+                        big = true;
                     }
-                    for (int pa = cs; pa < ce; pa++) {
-                        const float qx = S.sx[pa];
-                        const float qy = dim > 1 ? S.sy[pa] : 0.f;
-                        const float qz = dim > 2 ? S.sz[pa] : 0.f;
-                        const bool hit = valid && sqd(dim, qx, qy, qz, bx, by, bz) < e2;
-                        if (__ballot(hit) != 0ull) {
-                            found = true;
-                            break;
+                }
+            }
+            __syncthreads();
+            unite_if(S, ebuf, ecnt, touch, sA, touch ? slot_of(S, fcB) : 0);
+            // large candidate cells of this batch: one at a time, lanes over
+            // B's points, loop over A's cores, stop at the first pair within eps
+            uint64_t bigm = __ballot(big);
+            while (bigm) {
+                const int l = __ffsll((unsigned long long)bigm) - 1;
+                bigm &= bigm - 1;
+                const int Bl = lane_i(B, l);
+                const uint8_t cfB = S.cflag[Bl];
+                const int fcB = S.fcore[Bl];
+                const int csB = S.cstart[Bl], ceB = S.cstart[Bl + 1];
+                int rel = 0;
+                if (nA >= kBoxMin && ceB - csB >= kBoxMin) rel = box_relation(S, A, Bl);
+                bool touch = rel > 0;
+                if (rel == 0) {
+                    for (int b0 = csB; b0 < ceB && !touch; b0 += 64) {
+                        const int pb = b0 + lane;
+                        bool vb = pb < ceB;
+                        float bx = 0.f, by = 0.f, bz = 0.f;
+                        if (vb) {
+                            vb = (cfB & kCellDense) || (S.flags[pb] & kFlagCore);
+                            bx = S.sx[pb];
+                            if (dim > 1) by = S.sy[pb];
+                            if (dim > 2) bz = S.sz[pb];
+                        }
+                        for (int i = 0; i < nA; i++) {
+                            float qx, qy, qz;
+                            bool qc;
+                            if (inl) {
+                                qc = lc[i];
+                                qx = lx[i];
+                                qy = ly[i];
+                                qz = lz[i];
+                            } else {
+                                qc = (cfA & kCellDense) || (S.flags[csA + i] & kFlagCore);
+                                qx = S.sx[csA + i];
+                                qy = dim > 1 ? S.sy[csA + i] : 0.f;
+                                qz = dim > 2 ? S.sz[csA + i] : 0.f;
+                            }
+                            if (!qc) continue;
+                            if (__ballot(vb && sqd(dim, qx, qy, qz, bx, by, bz) < e2) != 0ull) {
+                                touch = true;
+                                break;
+                            }
                         }
                     }
                 }
-                if (found && lane == 0) uf_unite(S.parent, S.skey, sm, cs, cs2);
+                __syncthreads();
+                unite_if(S, ebuf, ecnt, touch && lane == 0, sA, slot_of(S, fcB));
             }
         }
     }
+    if (es >= 0) {
+        __syncthreads();
+        flush_edges(a.s[es], ebuf, ecnt);
+    }
 }
 
-// Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the root's slot.
+// Cluster merge, point level, for the cores of unsafe cells (grid border, or
+// grids whose cells are not dense-safe): each unites with every core within
+// eps (the reference's core branch of __intersection__hybrid_radius_sphere,
+// dbscan.cu:364-369), through the representatives of safe cells.
+__global__ __launch_bounds__(64) void k_union_points(BatchArgs a) {
+    const ChunkCtx x = chunk_of(a, blockIdx.x);
+    const StreamDev &S = a.s[x.s];
+    const int p = x.p0 + lane_id();
+    if (p >= x.p1) return;
+    if (!(S.flags[p] & kFlagCore)) return;
+    const int c = S.pcell[p];
+    if (S.cflag[c] & kCellSafe) return;  // k_union_cells
+    const int dim = S.dim;
+    const float e2 = S.eps2;
+    const float qx = S.sx[p];
+    const float qy = dim > 1 ? S.sy[p] : 0.f;
+    const float qz = dim > 2 ? S.sz[p] : 0.f;
+    const int2 *rp = S.rows + (size_t)c * S.R;
+    for (int r = 0; r < S.R; r++) {
+        const int2 rr = rp[r];
+        for (int j = rr.x; j < rr.y; j++) {
+            if (j == p) continue;
+            const float cx = S.sx[j];
+            const float cy = dim > 1 ? S.sy[j] : 0.f;
+            const float cz = dim > 2 ? S.sz[j] : 0.f;
+            if ((S.flags[j] & kFlagCore) && sqd(dim, qx, qy, qz, cx, cy, cz) < e2)
+                uf_unite(S.parent, slot_of(S, p), rep_slot(S, j));
+        }
+    }
+}
+
+// Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
+// root of the core's representative. Block 0 of each stream also advances the
+// ring cursor and hash epoch; no later kernel of the slide reads them.
 __global__ __launch_bounds__(64) void k_flatten(BatchArgs a) {
     const ChunkCtx x = chunk_of(a, blockIdx.x);
     const StreamDev &S = a.s[x.s];
     const IO io = a.use_io1 ? a.io1 : a.io[x.s];
     const int p = x.p0 + lane_id();
+    if (blockIdx.x == a.pre_chunk[x.s] && lane_id() == 0) {
+        StreamMut *m = S.mut;
+        m->t = m->t + 1;
+        m->u = m->u + 1 == S.units ? 0 : m->u + 1;
+        m->epoch = m->epoch >= kEpochMax ? 1u : m->epoch + 1u;
+    }
     if (p >= x.p1) return;
     if (!(S.flags[p] & kFlagCore)) return;
-    const uint32_t sm = slot_mask(S);
-    const int root = uf_find(S.parent, p);
-    const int v = (int)((uint32_t)S.skey[root] & sm);
+    const int v = uf_find(S.parent, rep_slot(S, p));  // minimum core slot of the cluster
     S.canon[p] = v;
-    const int slot = (int)((uint32_t)S.skey[p] & sm);
+    const int slot = slot_of(S, p);
     io.label[slot] = 0;
     io.cid[slot] = v;
 }
 
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
-// deterministic rule: the minimum canonical id among cores within eps).
-// Block 0 of each stream also advances the ring cursor and hash epoch; no
-// kernel of this launch reads them.
+// deterministic rule: the minimum canonical id among cores within eps), one
+// wave per cell with non-core points; candidates (coordinates and canonical
+// id, or none for non-cores) held in registers as in k_count.
 __global__ __launch_bounds__(64) void k_border(BatchArgs a) {
-    __shared__ float cx[kCap], cy[kCap], cz[kCap];
-    __shared__ int32_t cmeta[kCap];
-    const int g = blockIdx.x;
-    const ChunkCtx x = chunk_of(a, g);
-    const StreamDev &S = a.s[x.s];
-    const IO io = a.use_io1 ? a.io1 : a.io[x.s];
     const int lane = lane_id();
-    const int p = x.p0 + lane;
-    const bool mine = p < x.p1;
-    const int dim = S.dim;
-    const float e2 = S.eps2;
-    const uint32_t sm = slot_mask(S);
-    const bool mynon = mine && !(S.flags[p] & kFlagCore);
-    const int cfirst = S.pcell[x.p0], clast = S.pcell[x.p1 - 1];
-    int best = INT_MAX;
-    for (int c = cfirst; c <= clast; c++) {
-        const int cs = S.cstart[c], ce = S.cstart[c + 1];
-        if (is_dense(S, c)) continue;
-        const int q0 = max(cs, x.p0), q1 = min(ce, x.p1);
-        const bool qnon = mynon && p >= q0 && p < q1;
-        if (__ballot(qnon) == 0ull) continue;
-        for_candidates(
-            S, c, kCap,
-            [&](int i, int pos) {
-                cx[i] = S.sx[pos];
-                if (dim > 1) cy[i] = S.sy[pos];
-                if (dim > 2) cz[i] = S.sz[pos];
-                cmeta[i] = (S.flags[pos] & kFlagCore) ? S.canon[pos] : INT_MAX;
-            },
-            [&](int fill) {
-                const uint64_t qm = __ballot(qnon);
-                for (int jl = q0 - x.p0; jl < q1 - x.p0; jl++) {
-                    if (!((qm >> jl) & 1ull)) continue;
-                    const int pq = x.p0 + jl;
-                    const float qx = S.sx[pq];
-                    const float qy = dim > 1 ? S.sy[pq] : 0.f;
-                    const float qz = dim > 2 ? S.sz[pq] : 0.f;
-                    int bj = __shfl(best, jl);
-                    for (int i = 0; i < fill; i += 64) {
-                        const int idx = i + lane;
-                        int v = INT_MAX;
-                        if (idx < fill) {
-                            const int m = cmeta[idx];
-                            if (m != INT_MAX && sqd(dim, qx, qy, qz, cx[idx], cy[idx], cz[idx]) < e2) v = m;
-                        }
-                        if (__ballot(v != INT_MAX) != 0ull) bj = min(bj, wave_min(v));
+    const int total = a.cell_pre[a.n];
+    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
+        const CellCtx x = cell_of(a, gc);
+        const StreamDev &S = a.s[x.s];
+        const int c = x.c;
+        const uint8_t cf = S.cflag[c];
+        const int cs = S.cstart[c], ce = S.cstart[c + 1], nq = ce - cs;
+        const int2 myrow = load_row(S, c);
+        if (cf & kCellDense) continue;  // all core
+        const IO io = a.use_io1 ? a.io1 : a.io[x.s];
+        const int dim = S.dim, R = S.R;
+        const float e2 = S.eps2;
+        const uint32_t sm = slot_mask(S);
+        bool mine = false;
+        if (lane < nq) mine = !(S.flags[cs + lane] & kFlagCore);
+        bool any = mine;
+        for (int i = cs + 64 + lane; i < ce; i += 64) any = any || !(S.flags[i] & kFlagCore);
+        if (__ballot(any) == 0ull) continue;
+        const RowScan rs = row_scan_of(myrow);
+        const int T = rs.total;
+        for (int qb = 0; qb < nq; qb += 64) {
+            const int qn = min(64, nq - qb);
+            if (qb > 0) mine = lane < qn && !(S.flags[cs + qb + lane] & kFlagCore);
+            const uint64_t qm = __ballot(mine);
+            if (qm == 0ull) continue;
+            float mqx = 0.f, mqy = 0.f, mqz = 0.f;
+            if (mine) {
+                mqx = S.sx[cs + qb + lane];
+                if (dim > 1) mqy = S.sy[cs + qb + lane];
+                if (dim > 2) mqz = S.sz[cs + qb + lane];
+            }
+            int best = INT_MAX;
+            for (int cb = 0; cb < T; cb += 64 * kBatchM) {
+                float px[kBatchM], py[kBatchM], pz[kBatchM];
+                int meta[kBatchM];
+#pragma unroll
+                for (int m = 0; m < kBatchM; m++) {
+                    const int j = cb + 64 * m + lane;
+                    meta[m] = INT_MAX;
+                    px[m] = py[m] = pz[m] = 0.f;
+                    if (cb + 64 * m < T) {
+                        const int pos = cand_pos(rs, R, j < T ? j : T - 1);
+                        if (j < T && (S.flags[pos] & kFlagCore)) meta[m] = S.canon[pos];
+                        px[m] = S.sx[pos];
+                        if (dim > 1) py[m] = S.sy[pos];
+                        if (dim > 2) pz[m] = S.sz[pos];
                     }
-                    if (lane == jl) best = bj;
                 }
-                return true;
-            });
-    }
-    if (mynon) {
-        const int slot = (int)((uint32_t)S.skey[p] & sm);
-        io.label[slot] = best != INT_MAX ? 1 : 2;
-        io.cid[slot] = best != INT_MAX ? best : slot;
-    }
-    if (g == a.pre_chunk[x.s] && lane == 0) {
-        StreamMut *m = S.mut;
-        m->t = m->t + 1;
-        m->u = m->u + 1 == S.units ? 0 : m->u + 1;
-        m->epoch = m->epoch >= kEpochMax ? 1u : m->epoch + 1u;
+                uint64_t rem = qm;
+                while (rem) {
+                    const int q = __ffsll((unsigned long long)rem) - 1;
+                    rem &= rem - 1;
+                    const float qx = lane_f(mqx, q), qy = lane_f(mqy, q), qz = lane_f(mqz, q);
+                    int v = INT_MAX;
+#pragma unroll
+                    for (int m = 0; m < kBatchM; m++)
+                        if (meta[m] != INT_MAX && sqd(dim, qx, qy, qz, px[m], py[m], pz[m]) < e2)
+                            v = min(v, meta[m]);
+                    if (__ballot(v != INT_MAX) != 0ull) {
+                        v = wave_min(v);
+                        if (lane == q) best = min(best, v);
+                    }
+                }
+            }
+            if (mine) {
+                const int p = cs + qb + lane;
+                const int slot = (int)((uint32_t)S.skey[p] & sm);
+                io.label[slot] = best != INT_MAX ? 1 : 2;
+                io.cid[slot] = best != INT_MAX ? best : slot;
+            }
+        }
     }
 }
 
@@ -1015,15 +1367,16 @@ __global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
         if (e_ != hipSuccess) return e_;                              \
     } while (0)
 
-static inline hipError_t mark(hipEvent_t *ev, int idx, hipStream_t st) {
-    if (!ev) return hipSuccess;
+static inline hipError_t mark(hipEvent_t *ev, uint32_t mask, int idx, hipStream_t st) {
+    if (!ev || !((mask >> (idx / 2)) & 1u)) return hipSuccess;
     return hipEventRecord(ev[idx], st);
 }
 
-hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev) {
+hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev,
+                        uint32_t phase_mask) {
     hipError_t e;
-#define PH_BEGIN(ph) if ((e = mark(ev, 2 * (ph), st)) != hipSuccess) return e
-#define PH_END(ph) if ((e = mark(ev, 2 * (ph) + 1, st)) != hipSuccess) return e
+#define PH_BEGIN(ph) if ((e = mark(ev, phase_mask, 2 * (ph), st)) != hipSuccess) return e
+#define PH_END(ph) if ((e = mark(ev, phase_mask, 2 * (ph) + 1, st)) != hipSuccess) return e
     PH_BEGIN(kPhStrideSort);
     RTDBS_LAUNCH(k_stride_tiles<false>, sh.stiles, kTileThreads, a);
     PH_END(kPhStrideSort);
@@ -1044,17 +1397,17 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_rows, sh.persist_blocks, kTileThreads, a);
     PH_END(kPhRows);
     PH_BEGIN(kPhCount);
-    RTDBS_LAUNCH(k_count, sh.chunks, 64, a);
+    RTDBS_LAUNCH(k_count, sh.cell_blocks, 64, a);
     PH_END(kPhCount);
     PH_BEGIN(kPhUnion);
-    RTDBS_LAUNCH(k_union_sparse, sh.chunks, 64, a);
-    RTDBS_LAUNCH(k_union_dense, sh.persist_blocks * 4, 64, a);
+    RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
+    RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
     RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
-    RTDBS_LAUNCH(k_border, sh.chunks, 64, a);
+    RTDBS_LAUNCH(k_border, sh.cell_blocks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END
