@@ -111,6 +111,27 @@ def roofline(WL, res, sample=8):
             "avg_launch_us": round(avg_s * 1e6, 2)}
 
 
+def batched_roofline(WL, bres):
+    """Range-count roofline of the 64-stream launch: algorithmic bytes summed
+    over the streams' windows / the launch's HIP-event duration."""
+    from rtdbstream_amd import roofline as RF
+
+    avg_ms, ts, specs, datas, infos = bres.pop("_count_kernel")
+    total = 0.0
+    for sp, d, info in zip(specs, datas, infos):
+        W, S = sp["window"], sp["stride"]
+        per = []
+        for t in ts:
+            win = window_at(d, W, S, t)
+            q = ~RF.dense_mask(win, sp["eps"], sp["min_pts"], info)
+            per.append(RF.range_count_bytes(win, sp["eps"], q))
+        total += float(np.mean(per))
+    achieved = total / (avg_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_count (64-stream launch)",
+            "algorithmic_bytes_per_launch": int(total), "avg_launch_us": round(avg_ms * 1e3, 2)}
+
+
 def cpu_baseline(R, WL, res, seconds, max_slides=400):
     """CPU port (oracle grid, OpenMP) on a bounded sample of the same workload
     (about `seconds` of CPU work); the GPU labels of the same slides are
@@ -146,7 +167,7 @@ def cpu_baseline(R, WL, res, seconds, max_slides=400):
 
 
 def batched_leg(R, WL, n_streams, slides, warmup):
-    specs = WL.multi_stream_specs(n_streams=n_streams, slides=slides + warmup)
+    specs = WL.multi_stream_specs(n_streams=n_streams, slides=slides + warmup + 2)
     streams, bufs, datas = [], [], []
     for sp in specs:
         d = WL.stream_points(sp)
@@ -189,6 +210,7 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         s.init(d[:sp["window"]])
         streams.append(s)
     b = R.Batch(streams, use_graph=True)
+    bt = R.Batch(streams, use_graph=False)  # same streams, events around the range count
 
     def bslide(t):
         b.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
@@ -207,6 +229,7 @@ def batched_leg(R, WL, n_streams, slides, warmup):
     R.lib().rtdbs_device_synchronize()
     t_batch = time.perf_counter() - t0
     # exact: batched labels of the last slide equal the plain ones, per stream
+    # (checked before the timing pass below moves the windows on)
     same = True
     for i, sp in enumerate(specs):
         W = sp["window"]
@@ -216,7 +239,19 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         bufs[i][3].download(l2)
         bufs[i][4].download(c2)
         same = same and np.array_equal(l1, l2) and np.array_equal(c1, c2)
+    # range-count kernel of the batched launch: HIP events on the batch's own
+    # stream, two more slides without the graph
+    bt.set_timing(2)
+    t_extra = []
+    for t in range(warmup + slides, warmup + slides + 2):
+        bt.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                       [x[4].addr() for x in bufs])
+        bt.sync()
+        t_extra.append(t)
+    cnt_ms, cnt_n = bt.timing()["count"]
+    infos = [st.info() for st in streams]
     b.close()
+    bt.close()
     for s in streams:
         s.close()
     pts = sum(sp["stride"] for sp in specs) * slides
@@ -232,7 +267,84 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         "batched_step_p50_ms": round(float(np.percentile(lat_b, 50)), 3),
         "batched_step_p99_ms": round(float(np.percentile(lat_b, 99)), 3),
         "labels_equal_plain": bool(same),
+        "_count_kernel": (cnt_ms / max(1, cnt_n), t_extra, specs, datas, infos),
     }
+
+
+def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, seed=7):
+    """Request server under Poisson arrivals at `factors` x the serial plain
+    capacity: throughput and p50/p99 slide latency (submit -> labels in host
+    memory). Each stream's last result is compared with its plain result."""
+    specs = WL.served_stream_specs(n_streams=n_streams, slides=slides)
+    datas = [WL.stream_points(sp) for sp in specs]
+
+    def stride(i, t):
+        sp = specs[i]
+        return datas[i][sp["window"] + t * sp["stride"]: sp["window"] + (t + 1) * sp["stride"]]
+
+    # serial plain slides on the same GPU (host buffers, like the server)
+    plain_ms, last = [], []
+    for i, sp in enumerate(specs):
+        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+        s.init(datas[i][:sp["window"]])
+        for t in range(slides):
+            a = time.perf_counter()
+            l, c = s.slide(stride(i, t))
+            plain_ms.append((time.perf_counter() - a) * 1e3)
+        last.append((l.copy(), c.copy()))
+        s.close()
+    plain_total_s = sum(plain_ms) / 1e3
+    capacity = len(plain_ms) / plain_total_s
+    out = {"streams": n_streams, "slides_per_stream": slides,
+           "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
+           "hbm_budget_gb": budget_gb, "max_batch": max_batch, "lanes": lanes,
+           "plain_serial_slides_per_s": round(capacity, 1),
+           "plain_serial_points_per_s": round(sum(sp["stride"] for sp in specs) * slides / plain_total_s, 1),
+           "plain_p50_ms": round(float(np.percentile(plain_ms, 50)), 3),
+           "plain_p99_ms": round(float(np.percentile(plain_ms, 99)), 3), "sweep": []}
+    rng = np.random.default_rng(seed)
+    exact = True
+    for f in factors:
+        srv = R.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_batch=max_batch, lanes=lanes)
+        sids = [srv.open_stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"],
+                                datas[i][:sp["window"]]) for i, sp in enumerate(specs)]
+        order = np.repeat(np.arange(n_streams), slides)
+        rng.shuffle(order)
+        gaps = rng.exponential(1.0 / (f * capacity), size=len(order))
+        nxt = [0] * n_streams
+        tickets = []
+        t0 = time.perf_counter()
+        due = t0
+        for k, i in enumerate(order):
+            due += gaps[k]
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                if due - now > 2e-4:
+                    time.sleep((due - now) * 0.5)
+            t = nxt[i]
+            nxt[i] += 1
+            tickets.append((i, t, srv.submit(sids[i], stride(i, t))))
+        res_last = {}
+        for i, t, tk in tickets:
+            l, c = srv.wait(tk, 120000)
+            if t == slides - 1:
+                res_last[i] = (l, c)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        for i in range(n_streams):
+            exact = exact and np.array_equal(res_last[i][0], last[i][0]) and np.array_equal(res_last[i][1], last[i][1])
+        nreq = len(order)
+        out["sweep"].append({"rate_factor": f, "offered_slides_per_s": round(f * capacity, 1),
+                             "slides_per_s": round(nreq / wall, 1),
+                             "points_per_s": round(sum(specs[i]["stride"] for i in order) / wall, 1),
+                             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                             "mean_batch": round(st["mean_batch"], 2), "batches": st["batches"]})
+        log("served f=%.2f: %s" % (f, out["sweep"][-1]))
+    out["labels_equal_plain"] = bool(exact)
+    return out
 
 
 def main():
@@ -245,6 +357,10 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=64)
     ap.add_argument("--batch-slides", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-streams", type=int, default=256)
+    ap.add_argument("--serve-slides", type=int, default=3)
+    ap.add_argument("--serve-rates", default="0.5,1,2,4")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,7 +408,13 @@ def main():
     if not args.no_batch:
         log("batched leg: %d streams x %d slides" % (args.batch_streams, args.batch_slides))
         out["batched"] = batched_leg(R, WL, args.batch_streams, args.batch_slides, 2)
+        out["roofline_batched"] = batched_roofline(WL, out["batched"])
         log("batched: %s" % out["batched"])
+        log("roofline (batched launch): %s" % out["roofline_batched"])
+    if not args.no_serve:
+        log("served leg: %d streams x %d slides" % (args.serve_streams, args.serve_slides))
+        out["served"] = served_leg(R, WL, args.serve_streams, args.serve_slides,
+                                   [float(x) for x in args.serve_rates.split(",")], 64, 2, 64)
     print(json.dumps(out), flush=True)
 
 

@@ -27,7 +27,6 @@ constexpr uint8_t kFlagCore = 1;
 constexpr uint8_t kFlagDense = 2;
 constexpr uint8_t kCellSafe = 1;
 constexpr uint8_t kCellDense = 2;
-constexpr int kBoxMin = 16;  // cells at least this large get a bounding box
 constexpr int kNoCore = 0x7fffffff;  // fcore of a cell without cores
 
 // Per-stream state that the device itself advances every slide.
@@ -77,7 +76,6 @@ struct StreamDev {
     int2 *rows;       // W*R: candidate position ranges per cell
     uint8_t *cflag;   // W: kCellSafe | kCellDense per cell
     int32_t *fcore;   // W: first (minimum-slot) core position of each cell, kNoCore if none
-    float *cbox;      // 6W: per-cell bounding box (lo xyz, hi xyz) for cells of >= kBoxMin points
     // clustering
     uint8_t *flags;   // W: kFlagCore | kFlagDense
     int32_t *parent;  // W: union-find over positions, root = minimum slot
@@ -108,6 +106,7 @@ struct LaunchShape {
     int32_t passes;   // stride merge passes
     int32_t persist_blocks;  // 256-thread blocks for the persistent row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
+    int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
 };
 
 enum Phase {

@@ -111,7 +111,7 @@ int validate_params(const rtdbs_params *p) {
 struct Layout {
     size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
     size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
-    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cflag, off_fcore, off_cbox, off_flags, off_parent,
+    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cflag, off_fcore, off_flags, off_parent,
         off_canon, off_tile_a, off_tile_b, off_in, off_label, off_cid;
     size_t total;
 };
@@ -151,7 +151,6 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
     L.off_rows = take(W * (size_t)R * 8);
     L.off_cflag = take(W);
     L.off_fcore = take(W * 4);
-    L.off_cbox = take(W * 24);
     L.off_flags = take(W);
     L.off_parent = take(W * 4);
     L.off_canon = take(W * 4);
@@ -267,7 +266,6 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     D.rows = (int2 *)(A + L.off_rows);
     D.cflag = (uint8_t *)(A + L.off_cflag);
     D.fcore = (int32_t *)(A + L.off_fcore);
-    D.cbox = (float *)(A + L.off_cbox);
     D.flags = (uint8_t *)(A + L.off_flags);
     D.parent = (int32_t *)(A + L.off_parent);
     D.canon = (int32_t *)(A + L.off_canon);
@@ -572,6 +570,9 @@ int refresh_batch(rtdbs_batch *b) {
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
     b->shape.cell_blocks = (int)std::max(1ll, std::min((long long)device_cus() * 32, cells_total));
+    // lanes per query: enough threads to cover the device several times over
+    const long long lanes_wanted = (long long)device_cus() * 32 * 64 * 4;
+    b->shape.lpq = cells_total * 4 <= lanes_wanted ? 4 : (cells_total * 2 <= lanes_wanted ? 2 : 1);
     (void)cells_total;
     BatchArgs &a = b->args;
     std::memset(&a, 0, sizeof(a));

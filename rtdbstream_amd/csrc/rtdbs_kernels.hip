@@ -76,10 +76,7 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
-// Value of lane `l` (uniform) of a per-lane float/int.
-__device__ __forceinline__ float lane_f(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
+// Value of lane `l` (uniform) of a per-lane int.
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 // Sum over a 256-thread block.
@@ -770,36 +767,6 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
 // the device-side cell list in a persistent loop. The candidates of a cell
 // are the concatenation of its stencil rows; lanes 0..R-1 hold one row each.
 
-struct RowScan {
-    int rb;     // lane r: first window position of row r
-    int pre;    // lane r: candidates in rows before r
-    int total;  // uniform: candidates of the cell
-};
-
-__device__ __forceinline__ int2 load_row(const StreamDev &S, int c) {
-    int2 r = make_int2(0, 0);
-    if (lane_id() < S.R) r = S.rows[(size_t)c * S.R + lane_id()];
-    return r;
-}
-
-__device__ __forceinline__ RowScan row_scan_of(int2 r) {
-    const int lane = lane_id();
-    const int len = r.y - r.x;
-    int incl = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    RowScan rs;
-    rs.rb = r.x;
-    rs.pre = incl - len;
-    rs.total = __shfl(incl, 63);
-    return rs;
-}
-
-__device__ __forceinline__ RowScan row_scan(const StreamDev &S, int c) { return row_scan_of(load_row(S, c)); }
-
 // Index r of the row holding candidate j (largest r < R with pre(r) <= j).
 // Every lane takes part; j may differ per lane.
 __device__ __forceinline__ int row_of(int pre, int R, int j) {
@@ -816,11 +783,6 @@ __device__ __forceinline__ int row_of(int pre, int R, int j) {
     return lo;
 }
 
-__device__ __forceinline__ int cand_pos(const RowScan &rs, int R, int j) {
-    const int r = row_of(rs.pre, R, j);
-    return __shfl(rs.rb, r) + (j - __shfl(rs.pre, r));
-}
-
 struct CellCtx {
     int s, c;
 };
@@ -832,7 +794,6 @@ __device__ __forceinline__ CellCtx cell_of(const BatchArgs &a, int gc) {
     return x;
 }
 
-constexpr int kBatchM = 4;  // candidates held in registers per lane and pass
 
 struct ChunkCtx {
     int s, p0, p1;
@@ -856,173 +817,144 @@ __device__ __forceinline__ int rep_slot(const StreamDev &S, int j) {
     return slot_of(S, (S.cflag[cj] & kCellSafe) ? S.fcore[cj] : j);
 }
 
+constexpr int kRowGroup = 5;  // stencil rows fetched together
+constexpr int kUnroll = 4;    // candidates fetched together
+
+// Walks the stencil candidates of cell c for one query point, kRowGroup rows
+// and kUnroll candidates per memory round trip (a thread's walk is a chain of
+// dependent loads; batching shortens it). visit(j, x, y, z) returns true to
+// stop the walk.
+// With `step` > 1 the thread takes rows first, first+step, ... only (the
+// other rows go to sibling threads of the same query).
+// Candidates at positions >= limit are skipped (rows are position ranges).
+template <class Visit>
+__device__ __forceinline__ void walk_candidates(const StreamDev &S, int c, Visit visit, int first = 0,
+                                                int step = 1, int limit = 0x7fffffff) {
+    const int dim = S.dim, R = S.R;
+    const int2 *rp = S.rows + (size_t)c * R;
+    for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
+        int2 rr[kRowGroup];
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            rr[i] = (r0 + i * step < R) ? rp[r0 + i * step] : make_int2(0, 0);
+            rr[i].y = min(rr[i].y, limit);
+        }
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            for (int j = rr[i].x; j < rr[i].y; j += kUnroll) {
+                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    const int jj = min(j + u, rr[i].y - 1);
+                    cx[u] = S.sx[jj];
+                    cy[u] = dim > 1 ? S.sy[jj] : 0.f;
+                    cz[u] = dim > 2 ? S.sz[jj] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++)
+                    if (j + u < rr[i].y && visit(j + u, cx[u], cy[u], cz[u])) return;
+            }
+        }
+    }
+}
+
+// Query ownership of a block of a point-parallel kernel with LPQ lanes per
+// query: block g covers chunk g / LPQ; each query is served by LPQ adjacent
+// lanes that split its stencil rows. More lanes per query shorten each
+// thread's chain of dependent loads when a launch has few queries.
+template <int LPQ>
+struct QueryLane {
+    ChunkCtx x;
+    int p;    // window position of the query (may be >= x.p1: idle)
+    int sub;  // 0..LPQ-1
+};
+
+template <int LPQ>
+__device__ __forceinline__ QueryLane<LPQ> query_lane(const BatchArgs &a) {
+    QueryLane<LPQ> q;
+    const int chunk = blockIdx.x / LPQ, part = blockIdx.x % LPQ;
+    q.x = chunk_of(a, chunk);
+    q.sub = lane_id() % LPQ;
+    q.p = q.x.p0 + part * (64 / LPQ) + lane_id() / LPQ;
+    return q;
+}
+
+template <int LPQ>
+__device__ __forceinline__ int group_sum(int v) {
+#pragma unroll
+    for (int o = 1; o < LPQ; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <int LPQ>
+__device__ __forceinline__ int group_min(int v) {
+#pragma unroll
+    for (int o = 1; o < LPQ; o <<= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
 // Range count = core identification (the reference's __raygen__grid +
 // __intersection__grid, dbscan.cu:155-194,250-270, and set_label,
-// aabb.cu:575-587), one wave per cell. Dense cells are core without counting
-// (all pairs within eps). Otherwise each lane holds up to kBatchM stencil
-// candidates in registers and every query of the cell is tested against all
-// of them at once, neighbours counted with ballot/popcount up to min_pts.
-// The cell's first core becomes its union-find representative; cells of
-// >= kBoxMin points also record their bounding box.
+// aabb.cu:575-587). Point-parallel: dense cells are core without counting
+// (all pairs within eps); otherwise the LPQ lanes of a query count its
+// neighbours (itself included) over its cell's stencil rows, each stopping
+// once its own count reaches min_pts. Queries of one cell are adjacent and
+// read the same candidates.
+template <int LPQ>
 __global__ __launch_bounds__(64) void k_count(BatchArgs a) {
-    const int lane = lane_id();
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a);
+    const StreamDev &S = a.s[ql.x.s];
+    const int p = ql.p;
+    const bool valid = p < ql.x.p1;
+    const int dim = S.dim, minpts = S.min_pts;
+    const float e2 = S.eps2;
+    int cnt = 0;
+    bool dense = false;
+    int c = 0;
+    if (valid) {
+        c = S.pcell[p];
+        dense = S.cflag[c] & kCellDense;
+    }
+    if (valid && !dense) {
+        const float qx = S.sx[p];
+        const float qy = dim > 1 ? S.sy[p] : 0.f;
+        const float qz = dim > 2 ? S.sz[p] : 0.f;
+        walk_candidates(
+            S, c,
+            [&](int, float cx, float cy, float cz) {
+                cnt += sqd(dim, qx, qy, qz, cx, cy, cz) < e2;
+                return cnt >= minpts;
+            },
+            ql.sub, LPQ);
+    }
+    cnt = group_sum<LPQ>(cnt);
+    if (!valid || ql.sub != 0) return;
+    const int sl = slot_of(S, p);
+    S.parent[sl] = sl;
+    S.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
+}
+
+// First (minimum-slot) core of every cell - its union-find representative -
+// one thread per cell.
+__global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
     const int total = a.cell_pre[a.n];
-    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
+    for (int gc = blockIdx.x * kTileThreads + threadIdx.x; gc < total; gc += gridDim.x * kTileThreads) {
         const CellCtx x = cell_of(a, gc);
         const StreamDev &S = a.s[x.s];
         const int c = x.c;
         const int cs = S.cstart[c], ce = S.cstart[c + 1];
-        const uint8_t cf = S.cflag[c];
-        const int nq = ce - cs;
-        const int2 myrow = load_row(S, c);
-        const int dim = S.dim, minpts = S.min_pts;
-        const float e2 = S.eps2;
-        // lane q holds query q of the current block of 64
-        float mqx = 0.f, mqy = 0.f, mqz = 0.f;
-        if (lane < nq) {
-            mqx = S.sx[cs + lane];
-            if (dim > 1) mqy = S.sy[cs + lane];
-            if (dim > 2) mqz = S.sz[cs + lane];
-        }
-        if (nq >= kBoxMin) {
-            float mn[3], mx[3];
-            const float v0[3] = {mqx, mqy, mqz};
-            for (int d = 0; d < 3; d++) {
-                mn[d] = lane < nq ? v0[d] : INFINITY;
-                mx[d] = lane < nq ? v0[d] : -INFINITY;
-            }
-            for (int i = cs + 64 + lane; i < ce; i += 64) {
-                const float v[3] = {S.sx[i], dim > 1 ? S.sy[i] : 0.f, dim > 2 ? S.sz[i] : 0.f};
-                for (int d = 0; d < 3; d++) {
-                    mn[d] = fminf(mn[d], v[d]);
-                    mx[d] = fmaxf(mx[d], v[d]);
-                }
-            }
-            for (int d = 0; d < 3; d++) {
-                for (int o = 32; o > 0; o >>= 1) {
-                    mn[d] = fminf(mn[d], __shfl_xor(mn[d], o));
-                    mx[d] = fmaxf(mx[d], __shfl_xor(mx[d], o));
-                }
-            }
-            if (lane < 6) {
-                float v = mn[0];
-                if (lane == 1) v = mn[1];
-                if (lane == 2) v = mn[2];
-                if (lane == 3) v = mx[0];
-                if (lane == 4) v = mx[1];
-                if (lane == 5) v = mx[2];
-                S.cbox[(size_t)c * 6 + lane] = v;
-            }
-        }
-        if (cf & kCellDense) {
-            for (int i = lane; i < nq; i += 64) {
-                S.flags[cs + i] = kFlagCore | kFlagDense;
-                const int sl = slot_of(S, cs + i);
-                S.parent[sl] = sl;
-            }
-            if (lane == 0) S.fcore[c] = cs;  // first position of the cell holds its minimum slot
-            continue;
-        }
-        const RowScan rs = row_scan_of(myrow);
-        const int T = rs.total, R = S.R;
         int fc = kNoCore;
-        for (int qb = 0; qb < nq; qb += 64) {
-            const int qn = min(64, nq - qb);
-            if (qb > 0) {
-                mqx = mqy = mqz = 0.f;
-                if (lane < qn) {
-                    mqx = S.sx[cs + qb + lane];
-                    if (dim > 1) mqy = S.sy[cs + qb + lane];
-                    if (dim > 2) mqz = S.sz[cs + qb + lane];
+        if (S.cflag[c] & kCellDense) {
+            fc = cs;
+        } else {
+            for (int i = cs; i < ce; i++)
+                if (S.flags[i] & kFlagCore) {
+                    fc = i;
+                    break;
                 }
-            }
-            int mycnt = 0;
-            for (int cb = 0; cb < T; cb += 64 * kBatchM) {
-                float px[kBatchM], py[kBatchM], pz[kBatchM];
-                bool v[kBatchM];
-#pragma unroll
-                for (int m = 0; m < kBatchM; m++) {
-                    const int j = cb + 64 * m + lane;
-                    v[m] = j < T;
-                    px[m] = py[m] = pz[m] = 0.f;
-                    if (cb + 64 * m < T) {
-                        const int pos = cand_pos(rs, R, v[m] ? j : T - 1);
-                        px[m] = S.sx[pos];
-                        if (dim > 1) py[m] = S.sy[pos];
-                        if (dim > 2) pz[m] = S.sz[pos];
-                    }
-                }
-                for (int q = 0; q < qn; q++) {
-                    int cj = lane_i(mycnt, q);
-                    if (cj >= minpts) continue;
-                    const float qx = lane_f(mqx, q), qy = lane_f(mqy, q), qz = lane_f(mqz, q);
-#pragma unroll
-                    for (int m = 0; m < kBatchM; m++) {
-                        if (cb + 64 * m < T) {
-                            const bool hit = v[m] && sqd(dim, qx, qy, qz, px[m], py[m], pz[m]) < e2;
-                            cj += __popcll(__ballot(hit));
-                        }
-                    }
-                    if (lane == q) mycnt = cj;
-                }
-                if (__ballot(lane < qn && mycnt < minpts) == 0ull) break;
-            }
-            const bool core = lane < qn && mycnt >= minpts;
-            if (fc == kNoCore) {
-                const uint64_t m = __ballot(core);
-                if (m) fc = cs + qb + __ffsll((unsigned long long)m) - 1;
-            }
-            if (lane < qn) {
-                const int p = cs + qb + lane;
-                S.flags[p] = core ? kFlagCore : 0;
-                const int sl = slot_of(S, p);
-                S.parent[sl] = sl;
-            }
         }
-        if (lane == 0) S.fcore[c] = fc;
+        S.fcore[c] = fc;
     }
-}
-
-// Conservative bounding-box tests between two cells (double arithmetic): a
-// pair passing the FP32 test has true distance below eps*(1 + 2^-21).
-__device__ __forceinline__ int box_relation(const StreamDev &S, int A, int B) {
-    double mn2 = 0.0, mx2 = 0.0;
-    for (int d = 0; d < S.dim; d++) {
-        const double alo = S.cbox[(size_t)A * 6 + d], ahi = S.cbox[(size_t)A * 6 + 3 + d];
-        const double blo = S.cbox[(size_t)B * 6 + d], bhi = S.cbox[(size_t)B * 6 + 3 + d];
-        const double gap = fmax(blo - ahi, alo - bhi);
-        if (gap > 0.0) mn2 += gap * gap;
-        const double span = fmax(bhi - alo, ahi - blo);
-        mx2 += span * span;
-    }
-    const double e2 = (double)S.eps2;
-    if (mn2 >= e2 * (1.0 + 1.0 / 1048576.0)) return -1;  // no pair within eps
-    if (mx2 * (1.0 + 1.0 / 1048576.0) < e2) return 1;    // every pair within eps
-    return 0;
-}
-
-// True if some core of cell A and some core of cell B are within eps.
-__device__ __forceinline__ bool cells_touch(const StreamDev &S, int A, int B, uint8_t cfA, uint8_t cfB) {
-    const int csA = S.cstart[A], ceA = S.cstart[A + 1];
-    const int csB = S.cstart[B], ceB = S.cstart[B + 1];
-    if (ceA - csA >= kBoxMin && ceB - csB >= kBoxMin) {
-        const int rel = box_relation(S, A, B);
-        if (rel < 0) return false;
-        if (rel > 0) return true;  // both cells hold a core
-    }
-    const int dim = S.dim;
-    const float e2 = S.eps2;
-    const bool allA = cfA & kCellDense, allB = cfB & kCellDense;
-    for (int i = csA; i < ceA; i++) {
-        if (!allA && !(S.flags[i] & kFlagCore)) continue;
-        const float ax = S.sx[i], ay = dim > 1 ? S.sy[i] : 0.f, az = dim > 2 ? S.sz[i] : 0.f;
-        for (int j = csB; j < ceB; j++) {
-            if (!allB && !(S.flags[j] & kFlagCore)) continue;
-            if (sqd(dim, ax, ay, az, S.sx[j], dim > 1 ? S.sy[j] : 0.f, dim > 2 ? S.sz[j] : 0.f) < e2)
-                return true;
-        }
-    }
-    return false;
 }
 
 // Per-wave edge buffer in LDS: touching pairs are collected while the wave
@@ -1167,10 +1099,8 @@ __global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
                 const uint8_t cfB = S.cflag[Bl];
                 const int fcB = S.fcore[Bl];
                 const int csB = S.cstart[Bl], ceB = S.cstart[Bl + 1];
-                int rel = 0;
-                if (nA >= kBoxMin && ceB - csB >= kBoxMin) rel = box_relation(S, A, Bl);
-                bool touch = rel > 0;
-                if (rel == 0) {
+                bool touch = false;
+                {
                     for (int b0 = csB; b0 < ceB && !touch; b0 += 64) {
                         const int pb = b0 + lane;
                         bool vb = pb < ceB;
@@ -1231,18 +1161,12 @@ __global__ __launch_bounds__(64) void k_union_points(BatchArgs a) {
     const float qx = S.sx[p];
     const float qy = dim > 1 ? S.sy[p] : 0.f;
     const float qz = dim > 2 ? S.sz[p] : 0.f;
-    const int2 *rp = S.rows + (size_t)c * S.R;
-    for (int r = 0; r < S.R; r++) {
-        const int2 rr = rp[r];
-        for (int j = rr.x; j < rr.y; j++) {
-            if (j == p) continue;
-            const float cx = S.sx[j];
-            const float cy = dim > 1 ? S.sy[j] : 0.f;
-            const float cz = dim > 2 ? S.sz[j] : 0.f;
-            if ((S.flags[j] & kFlagCore) && sqd(dim, qx, qy, qz, cx, cy, cz) < e2)
-                uf_unite(S.parent, slot_of(S, p), rep_slot(S, j));
-        }
-    }
+    const int sp = slot_of(S, p);
+    walk_candidates(S, c, [&](int j, float cx, float cy, float cz) {
+        if (j != p && sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
+            uf_unite(S.parent, sp, rep_slot(S, j));
+        return false;
+    });
 }
 
 // Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
@@ -1269,83 +1193,38 @@ __global__ __launch_bounds__(64) void k_flatten(BatchArgs a) {
 }
 
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
-// deterministic rule: the minimum canonical id among cores within eps), one
-// wave per cell with non-core points; candidates (coordinates and canonical
-// id, or none for non-cores) held in registers as in k_count.
+// deterministic rule: the minimum canonical id among cores within eps),
+// point-parallel over the non-core points, LPQ lanes per point.
+template <int LPQ>
 __global__ __launch_bounds__(64) void k_border(BatchArgs a) {
-    const int lane = lane_id();
-    const int total = a.cell_pre[a.n];
-    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
-        const CellCtx x = cell_of(a, gc);
-        const StreamDev &S = a.s[x.s];
-        const int c = x.c;
-        const uint8_t cf = S.cflag[c];
-        const int cs = S.cstart[c], ce = S.cstart[c + 1], nq = ce - cs;
-        const int2 myrow = load_row(S, c);
-        if (cf & kCellDense) continue;  // all core
-        const IO io = a.use_io1 ? a.io1 : a.io[x.s];
-        const int dim = S.dim, R = S.R;
-        const float e2 = S.eps2;
-        const uint32_t sm = slot_mask(S);
-        bool mine = false;
-        if (lane < nq) mine = !(S.flags[cs + lane] & kFlagCore);
-        bool any = mine;
-        for (int i = cs + 64 + lane; i < ce; i += 64) any = any || !(S.flags[i] & kFlagCore);
-        if (__ballot(any) == 0ull) continue;
-        const RowScan rs = row_scan_of(myrow);
-        const int T = rs.total;
-        for (int qb = 0; qb < nq; qb += 64) {
-            const int qn = min(64, nq - qb);
-            if (qb > 0) mine = lane < qn && !(S.flags[cs + qb + lane] & kFlagCore);
-            const uint64_t qm = __ballot(mine);
-            if (qm == 0ull) continue;
-            float mqx = 0.f, mqy = 0.f, mqz = 0.f;
-            if (mine) {
-                mqx = S.sx[cs + qb + lane];
-                if (dim > 1) mqy = S.sy[cs + qb + lane];
-                if (dim > 2) mqz = S.sz[cs + qb + lane];
-            }
-            int best = INT_MAX;
-            for (int cb = 0; cb < T; cb += 64 * kBatchM) {
-                float px[kBatchM], py[kBatchM], pz[kBatchM];
-                int meta[kBatchM];
-#pragma unroll
-                for (int m = 0; m < kBatchM; m++) {
-                    const int j = cb + 64 * m + lane;
-                    meta[m] = INT_MAX;
-                    px[m] = py[m] = pz[m] = 0.f;
-                    if (cb + 64 * m < T) {
-                        const int pos = cand_pos(rs, R, j < T ? j : T - 1);
-                        if (j < T && (S.flags[pos] & kFlagCore)) meta[m] = S.canon[pos];
-                        px[m] = S.sx[pos];
-                        if (dim > 1) py[m] = S.sy[pos];
-                        if (dim > 2) pz[m] = S.sz[pos];
-                    }
-                }
-                uint64_t rem = qm;
-                while (rem) {
-                    const int q = __ffsll((unsigned long long)rem) - 1;
-                    rem &= rem - 1;
-                    const float qx = lane_f(mqx, q), qy = lane_f(mqy, q), qz = lane_f(mqz, q);
-                    int v = INT_MAX;
-#pragma unroll
-                    for (int m = 0; m < kBatchM; m++)
-                        if (meta[m] != INT_MAX && sqd(dim, qx, qy, qz, px[m], py[m], pz[m]) < e2)
-                            v = min(v, meta[m]);
-                    if (__ballot(v != INT_MAX) != 0ull) {
-                        v = wave_min(v);
-                        if (lane == q) best = min(best, v);
-                    }
-                }
-            }
-            if (mine) {
-                const int p = cs + qb + lane;
-                const int slot = (int)((uint32_t)S.skey[p] & sm);
-                io.label[slot] = best != INT_MAX ? 1 : 2;
-                io.cid[slot] = best != INT_MAX ? best : slot;
-            }
-        }
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a);
+    const StreamDev &S = a.s[ql.x.s];
+    const int p = ql.p;
+    bool mine = p < ql.x.p1;
+    if (mine) mine = !(S.flags[p] & kFlagCore);
+    const int dim = S.dim;
+    const float e2 = S.eps2;
+    int best = INT_MAX;
+    if (mine) {
+        const int c = S.pcell[p];
+        const float qx = S.sx[p];
+        const float qy = dim > 1 ? S.sy[p] : 0.f;
+        const float qz = dim > 2 ? S.sz[p] : 0.f;
+        walk_candidates(
+            S, c,
+            [&](int j, float cx, float cy, float cz) {
+                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
+                    best = min(best, S.canon[j]);
+                return false;
+            },
+            ql.sub, LPQ);
     }
+    best = group_min<LPQ>(best);
+    if (!mine || ql.sub != 0) return;
+    const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
+    const int slot = slot_of(S, p);
+    io.label[slot] = best != INT_MAX ? 1 : 2;
+    io.cid[slot] = best != INT_MAX ? best : slot;
 }
 
 __global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
@@ -1397,9 +1276,12 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_rows, sh.persist_blocks, kTileThreads, a);
     PH_END(kPhRows);
     PH_BEGIN(kPhCount);
-    RTDBS_LAUNCH(k_count, sh.cell_blocks, 64, a);
+    if (sh.lpq >= 4) RTDBS_LAUNCH(k_count<4>, sh.chunks * 4, 64, a);
+    else if (sh.lpq == 2) RTDBS_LAUNCH(k_count<2>, sh.chunks * 2, 64, a);
+    else RTDBS_LAUNCH(k_count<1>, sh.chunks, 64, a);
     PH_END(kPhCount);
     PH_BEGIN(kPhUnion);
+    RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
     RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
     RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
     PH_END(kPhUnion);
@@ -1407,7 +1289,9 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
-    RTDBS_LAUNCH(k_border, sh.cell_blocks, 64, a);
+    if (sh.lpq >= 4) RTDBS_LAUNCH(k_border<4>, sh.chunks * 4, 64, a);
+    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border<2>, sh.chunks * 2, 64, a);
+    else RTDBS_LAUNCH(k_border<1>, sh.chunks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END
