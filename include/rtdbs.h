@@ -59,8 +59,9 @@ typedef struct rtdbs_params {
     int32_t min_pts;  /* MinPts, self included (reference -K, util.cpp:480; aabb.cu:153) */
     float lo[3];      /* grid bounds (reference: whole-file min/max, util.cpp:23-55);   */
     float hi[3];      /* if hi[a] <= lo[a] on a used axis they come from the first window */
-    int32_t cell_div; /* stencil reach k (grid side = Eps/k), 1..3; 0 = autotuned default */
-    int32_t reserved[7];
+    int32_t cell_div; /* stencil reach k (grid side = Eps/k), 1..3; 0 = default (2) */
+    int32_t lanes_per_query; /* threads per query point in the plain path: 1, 2, 4; 0 = auto */
+    int32_t reserved[6];
 } rtdbs_params;
 
 typedef struct rtdbs_stream rtdbs_stream;
@@ -131,6 +132,26 @@ int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *stride_points_de
                             int32_t *const *labels_dev, int32_t *const *cids_dev);
 int rtdbs_batch_sync(rtdbs_batch *b);
 void rtdbs_batch_destroy(rtdbs_batch *b);
+
+/* ---- autotuner ------------------------------------------------------------ */
+/* No reference counterpart (the reference fixes the grid side to Eps/sqrt(3),
+ * dbscan.cpp:34). Times trial slides of the given stream sample for every
+ * stencil reach k (grid side Eps/k) and lanes-per-query setting and returns
+ * the fastest. `points` holds W + slides*S points (slides >= 1). Results are
+ * cached by (dim, W, S, MinPts, density) in the file named by the
+ * RTDBS_TUNE_CACHE environment variable (default
+ * $HOME/.cache/rtdbs_autotune.txt); a cache hit runs no trials. */
+typedef struct rtdbs_tune_result {
+    int32_t cell_div;
+    int32_t lanes_per_query;
+    double ms_per_slide;   /* of the chosen setting (from the trials or the cache) */
+    int32_t from_cache;
+    int32_t trials;
+    char key[96];
+} rtdbs_tune_result;
+
+int rtdbs_autotune(const rtdbs_params *params, const float *points, int64_t n_points, int32_t slides,
+                   rtdbs_tune_result *out);
 
 /* ---- request server (continuous batching, HBM-budget admission) ---------- */
 /* No reference counterpart. */

@@ -16,7 +16,7 @@ import numpy as np
 
 __all__ = [
     "RtdbsError", "Params", "Stream", "Batch", "Server", "DeviceBuffer", "lib", "library_path",
-    "LABEL_CORE", "LABEL_BORDER", "LABEL_NOISE", "device_count",
+    "LABEL_CORE", "LABEL_BORDER", "LABEL_NOISE", "device_count", "autotune",
 ]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -43,7 +43,14 @@ class Params(ctypes.Structure):
         ("dim", ctypes.c_int32), ("window", ctypes.c_int32), ("stride", ctypes.c_int32),
         ("eps", ctypes.c_float), ("min_pts", ctypes.c_int32),
         ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
-        ("cell_div", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+    ]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("ms_per_slide", ctypes.c_double),
+        ("from_cache", ctypes.c_int32), ("trials", ctypes.c_int32), ("key", ctypes.c_char * 96),
     ]
 
 
@@ -117,6 +124,8 @@ SIGNATURES = [
     ("rtdbs_stream_get_timing", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                                ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     ("rtdbs_test_fail_allocations", ctypes.c_int, [ctypes.c_int]),
+    ("rtdbs_autotune", ctypes.c_int, [ctypes.POINTER(Params), _P, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.POINTER(TuneResult)]),
     ("rtdbs_last_error", ctypes.c_char_p, []),
     ("rtdbs_version", ctypes.c_char_p, []),
 ]
@@ -155,14 +164,26 @@ def device_count():
     return n.value
 
 
-def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0):
+def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0):
     p = Params()
     p.dim, p.window, p.stride, p.eps, p.min_pts = int(dim), int(window), int(stride), float(eps), int(min_pts)
     for a in range(3):
         p.lo[a] = float(lo[a]) if lo is not None and a < len(lo) else 0.0
         p.hi[a] = float(hi[a]) if hi is not None and a < len(hi) else 0.0
     p.cell_div = int(cell_div)
+    p.lanes_per_query = int(lanes_per_query)
     return p
+
+
+def autotune(dim, window, stride, eps, min_pts, points, slides=3, lo=None, hi=None):
+    """Stencil reach and lanes-per-query for this stream, from timed trial
+    slides on `points` (W + (slides+1)*S points) or the tuning cache."""
+    p = make_params(dim, window, stride, eps, min_pts, lo, hi)
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    r = TuneResult()
+    _check(lib().rtdbs_autotune(ctypes.byref(p), _ptr(pts), pts.size // int(dim), int(slides), ctypes.byref(r)))
+    return {"cell_div": r.cell_div, "lanes_per_query": r.lanes_per_query, "ms_per_slide": r.ms_per_slide,
+            "from_cache": bool(r.from_cache), "trials": r.trials, "key": r.key.decode()}
 
 
 def _f32(a, n, dim, what):
@@ -213,8 +234,8 @@ class DeviceBuffer:
 class Stream:
     """One sliding window (the reference's ScanState + slide loop body)."""
 
-    def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0):
-        self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div)
+    def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0):
+        self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div, lanes_per_query)
         self.dim, self.W, self.S = int(dim), int(window), int(stride)
         h = ctypes.c_void_p()
         _check(lib().rtdbs_stream_create(ctypes.byref(self.params), ctypes.byref(h)))

@@ -101,6 +101,8 @@ int validate_params(const rtdbs_params *p) {
     if (p->min_pts < 1) return fail(RTDBS_EINVAL, "min_pts must be >= 1 (got %d)", p->min_pts);
     if (p->cell_div < 0 || p->cell_div > kMaxK)
         return fail(RTDBS_EINVAL, "cell_div must be 0 (auto) or 1..%d (got %d)", kMaxK, p->cell_div);
+    if (!(p->lanes_per_query == 0 || p->lanes_per_query == 1 || p->lanes_per_query == 2 || p->lanes_per_query == 4))
+        return fail(RTDBS_EINVAL, "lanes_per_query must be 0 (auto), 1, 2 or 4 (got %d)", p->lanes_per_query);
     for (int a = 0; a < p->dim; a++)
         if (!std::isfinite(p->lo[a]) || !std::isfinite(p->hi[a]))
             return fail(RTDBS_EINVAL, "grid bounds must be finite");
@@ -570,9 +572,11 @@ int refresh_batch(rtdbs_batch *b) {
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
     b->shape.cell_blocks = (int)std::max(1ll, std::min((long long)device_cus() * 32, cells_total));
-    // lanes per query: enough threads to cover the device several times over
+    // lanes per query: enough threads to cover the device several times over,
+    // unless a single stream asks for a tuned setting
     const long long lanes_wanted = (long long)device_cus() * 32 * 64 * 4;
     b->shape.lpq = cells_total * 4 <= lanes_wanted ? 4 : (cells_total * 2 <= lanes_wanted ? 2 : 1);
+    if (n == 1 && b->streams[0]->p.lanes_per_query > 0) b->shape.lpq = b->streams[0]->p.lanes_per_query;
     (void)cells_total;
     BatchArgs &a = b->args;
     std::memset(&a, 0, sizeof(a));

@@ -1,0 +1,176 @@
+// Autotuner: picks the grid side (stencil reach k, side = Eps/k) and the
+// lanes-per-query setting of the point-parallel kernels by timing trial slides
+// on a sample of the stream, and remembers the choice by a workload signature.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include <sys/stat.h>
+
+#include "rtdbs.h"
+#include "rtdbs_host.h"
+
+using namespace rtdbs;
+
+namespace {
+
+std::mutex g_tune_mu;
+
+std::string cache_path() {
+    const char *e = std::getenv("RTDBS_TUNE_CACHE");
+    if (e && *e) return e;
+    const char *h = std::getenv("HOME");
+    return std::string(h && *h ? h : "/tmp") + "/.cache/rtdbs_autotune.txt";
+}
+
+// Point-weighted mean occupancy of Eps-side cells of the first window: the
+// density a query sees, which decides between fine and coarse grids.
+double occupancy(const rtdbs_params &p, const float *pts) {
+    const int d = p.dim;
+    std::unordered_map<uint64_t, int> cnt;
+    cnt.reserve((size_t)p.window * 2);
+    const double inv = 1.0 / (double)p.eps;
+    for (int i = 0; i < p.window; i++) {
+        uint64_t key = 1469598103934665603ull;
+        for (int a = 0; a < d; a++) {
+            const double v = std::floor((double)pts[(size_t)i * d + a] * inv);
+            const int64_t iv = std::isfinite(v) ? (int64_t)v : 0;
+            key = (key ^ (uint64_t)iv) * 1099511628211ull;
+        }
+        cnt[key]++;
+    }
+    double s2 = 0.0;
+    for (auto &kv : cnt) s2 += (double)kv.second * kv.second;
+    return s2 / (double)p.window;
+}
+
+std::string make_key(const rtdbs_params &p, const float *pts) {
+    const double occ = occupancy(p, pts);
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "d%d_w%d_s%d_m%d_o%d", p.dim, (int)std::lround(std::log2((double)p.window)),
+                  (int)std::lround(std::log2((double)p.window / p.stride)), p.min_pts,
+                  (int)std::lround(2.0 * std::log2(std::max(occ, 1e-3))));
+    return buf;
+}
+
+bool cache_lookup(const std::string &key, int *k, int *lpq, double *ms) {
+    FILE *f = std::fopen(cache_path().c_str(), "r");
+    if (!f) return false;
+    char kb[128];
+    int kk, ll;
+    double mm;
+    bool hit = false;
+    while (std::fscanf(f, "%127s %d %d %lf", kb, &kk, &ll, &mm) == 4) {
+        if (key == kb) {  // last entry wins
+            *k = kk;
+            *lpq = ll;
+            *ms = mm;
+            hit = true;
+        }
+    }
+    std::fclose(f);
+    return hit;
+}
+
+void cache_store(const std::string &key, int k, int lpq, double ms) {
+    const std::string path = cache_path();
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) {
+        const std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); i++)
+            if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f) return;  // read-only home: tuning still returns its result
+    std::fprintf(f, "%s %d %d %.6f\n", key.c_str(), k, lpq, ms);
+    std::fclose(f);
+}
+
+// Milliseconds per slide of `slides` timed slides (after one warm-up slide).
+int trial(const rtdbs_params &p, const float *pts, int slides, double *ms) {
+    rtdbs_stream *s = nullptr;
+    int rc = rtdbs_stream_create(&p, &s);
+    if (rc) return rc;
+    rc = rtdbs_stream_init(s, pts);
+    const size_t W = (size_t)p.window, S = (size_t)p.stride, d = (size_t)p.dim;
+    std::vector<int32_t> label(W), cid(W);
+    if (!rc) rc = rtdbs_stream_slide(s, pts + W * d, label.data(), cid.data());
+    if (!rc) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int t = 1; t <= slides && !rc; t++)
+            rc = rtdbs_stream_slide(s, pts + (W + (size_t)t * S) * d, label.data(), cid.data());
+        auto t1 = std::chrono::steady_clock::now();
+        *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / slides;
+    }
+    rtdbs_stream_destroy(s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, int64_t n_points, int32_t slides,
+                              rtdbs_tune_result *out) {
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (!points || !out) return fail(RTDBS_EINVAL, "NULL argument");
+    if (slides < 1) return fail(RTDBS_EINVAL, "slides must be >= 1");
+    const rtdbs_params &p0 = *params;
+    if (n_points < (int64_t)p0.window + (int64_t)(slides + 1) * p0.stride)
+        return fail(RTDBS_EINVAL, "the sample needs W + (slides+1)*S = %lld points (got %lld)",
+                    (long long)p0.window + (long long)(slides + 1) * p0.stride, (long long)n_points);
+    std::memset(out, 0, sizeof(*out));
+    const std::string key = make_key(p0, points);
+    std::snprintf(out->key, sizeof(out->key), "%s", key.c_str());
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int k = 0, lpq = 0;
+    double ms = 0.0;
+    if (cache_lookup(key, &k, &lpq, &ms)) {
+        out->cell_div = k;
+        out->lanes_per_query = lpq;
+        out->ms_per_slide = ms;
+        out->from_cache = 1;
+        return RTDBS_OK;
+    }
+    double best = 1e300;
+    int bk = 2, bl = 0;
+    for (int kk = 1; kk <= kMaxK; kk++) {
+        rtdbs_params p = p0;
+        p.cell_div = kk;
+        p.lanes_per_query = 0;
+        double m = 0.0;
+        rc = trial(p, points, slides, &m);
+        if (rc) return rc;
+        out->trials++;
+        if (m < best) {
+            best = m;
+            bk = kk;
+        }
+    }
+    for (int ll : {1, 2, 4}) {
+        rtdbs_params p = p0;
+        p.cell_div = bk;
+        p.lanes_per_query = ll;
+        double m = 0.0;
+        rc = trial(p, points, slides, &m);
+        if (rc) return rc;
+        out->trials++;
+        if (m < best) {
+            best = m;
+            bl = ll;
+        }
+    }
+    out->cell_div = bk;
+    out->lanes_per_query = bl;
+    out->ms_per_slide = best;
+    cache_store(key, bk, bl, best);
+    return RTDBS_OK;
+}

@@ -122,3 +122,68 @@ def test_repeat_runs_identical(engine):
     b = engine_slides(engine, data, 3, 3000, 500, 0.3, 6, 4)
     for (l1, c1), (l2, c2) in zip(a, b):
         assert np.array_equal(l1, l2) and np.array_equal(c1, c2)
+
+
+def test_walks3d_full_config_first_slides(engine, oracle):
+    cfg = WL.WORKLOADS["walks3d"]
+    W, S, slides = cfg["window"], cfg["stride"], 4
+    data = WL.generate("walks3d", n=W + slides * S)
+    ref_l, ref_c = oracle.stream(data, W, S, cfg["eps"], cfg["min_pts"], slides, method="grid")
+    got = engine_slides(engine, data, 3, W, S, cfg["eps"], cfg["min_pts"], slides)
+    assert_parity(got, ref_l, ref_c, "walks3d")
+
+
+def test_skewed3d_full_window(engine, oracle):
+    """W = 1,000,000, S = 50,000, Eps 1, MinPts 20: two slides against the
+    oracle, then size-independent checks on later slides."""
+    cfg = WL.WORKLOADS["skewed3d"]
+    W, S = cfg["window"], cfg["stride"]
+    data = WL.generate("skewed3d", n=W + 4 * S)
+    ref_l, ref_c = oracle.stream(data, W, S, cfg["eps"], cfg["min_pts"], 2, method="grid")
+    s = engine.Stream(3, W, S, cfg["eps"], cfg["min_pts"])
+    s.init(data[:W])
+    for t in range(4):
+        l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        if t < 2:
+            assert_parity([(l, c)], ref_l[t:t + 1], ref_c[t:t + 1], "skewed3d slide %d" % t)
+        else:
+            # invariants: labels in range; noise keeps its slot; a core's id is a
+            # core slot no larger than its own; every id names a core
+            assert set(np.unique(l)) <= {0, 1, 2}
+            noise = np.flatnonzero(l == 2)
+            assert np.array_equal(c[noise], noise)
+            core = l == 0
+            assert np.all(c[core] <= np.flatnonzero(core))
+            assert np.all(l[c[l != 2]] == 0)
+            assert np.all(c[c[core]] == c[core])
+    s.close()
+
+
+def test_autotune_choice_is_exact(engine, oracle, tmp_path, monkeypatch):
+    monkeypatch.setenv("RTDBS_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    W, S, slides = 4000, 400, 3
+    data = _mixed(W + 8 * S, 3, 41)
+    r = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
+    assert r["cell_div"] in (1, 2, 3) and r["lanes_per_query"] in (1, 2, 4)
+    assert not r["from_cache"] and r["trials"] == 6 and r["ms_per_slide"] > 0
+    r2 = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
+    assert r2["from_cache"] and r2["cell_div"] == r["cell_div"] and r2["trials"] == 0
+    ref_l, ref_c = oracle.stream(data, W, S, 0.3, 6, 5, method="grid")
+    s = engine.Stream(3, W, S, 0.3, 6, cell_div=r["cell_div"], lanes_per_query=r["lanes_per_query"])
+    s.init(data[:W])
+    for t in range(5):
+        l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
+@pytest.mark.parametrize("lpq", [1, 2, 4])
+def test_lanes_per_query_settings_agree(engine, oracle, lpq):
+    data = _mixed(3000 + 4 * 300, 2, 42)
+    ref_l, ref_c = oracle.stream(data, 3000, 300, 0.2, 5, 4, method="grid")
+    s = engine.Stream(2, 3000, 300, 0.2, 5, lanes_per_query=lpq)
+    s.init(data[:3000])
+    for t in range(4):
+        l, c = s.slide(data[3000 + t * 300: 3000 + (t + 1) * 300])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
