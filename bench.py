@@ -361,6 +361,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=256)
     ap.add_argument("--serve-slides", type=int, default=3)
     ap.add_argument("--serve-rates", default="0.5,1,2,4")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this size (profiler byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -375,6 +377,12 @@ def main():
     if R.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        src, dst = R.DeviceBuffer(nb), R.DeviceBuffer(nb)
+        R._check(R.lib().rtdbs_device_copy(R.ctypes.c_void_p(dst.addr()), R.ctypes.c_void_p(src.addr()), nb))
+        log("calibration copy: %d MiB read + %d MiB written by k_copy16" % (args.calib_copy_mb, args.calib_copy_mb))
+        del src, dst
     log("plain leg: %s, %d warmup + %d timed slides" % (args.workload, args.warmup, args.steps))
     res = plain_leg(R, WL, args.workload, args.steps, args.warmup)
     W, S = res["W"], res["S"]

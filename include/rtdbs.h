@@ -209,6 +209,9 @@ int rtdbs_device_alloc(size_t bytes, void **ptr);
 int rtdbs_device_free(void *ptr);
 int rtdbs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
 int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+/* Device-to-device copy by a plain kernel (bytes a multiple of 16): a kernel
+ * of known traffic for scaling profiler byte counters. Synchronous. */
+int rtdbs_device_copy(void *dst_dev, const void *src_dev, size_t bytes);
 int rtdbs_device_synchronize(void);
 int rtdbs_device_count(int *count);
 

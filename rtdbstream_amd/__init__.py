@@ -115,6 +115,7 @@ SIGNATURES = [
     ("rtdbs_device_free", ctypes.c_int, [_P]),
     ("rtdbs_memcpy_h2d", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_memcpy_d2h", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("rtdbs_device_copy", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_device_synchronize", ctypes.c_int, []),
     ("rtdbs_device_count", ctypes.c_int, [_I32P]),
     ("rtdbs_batch_set_timing", ctypes.c_int, [_P, ctypes.c_int]),

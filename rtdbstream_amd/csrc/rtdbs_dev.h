@@ -130,6 +130,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
                         uint32_t phase_mask);
 // Enqueues the first-window load (sort of W points into the window arrays).
 hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipStream_t st);
+// Copies n16 16-byte words (profiler byte-counter calibration).
+hipError_t launch_copy16(const void *src, void *dst, size_t n16, hipStream_t st);
 // Clears the cell hash of every stream in the batch (epoch wrap).
 hipError_t launch_clear_hash(const BatchArgs &a, hipStream_t st);
 

@@ -877,6 +877,13 @@ extern "C" int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t byte
     return RTDBS_OK;
 }
 
+extern "C" int rtdbs_device_copy(void *dst_dev, const void *src_dev, size_t bytes) {
+    if (!dst_dev || !src_dev || bytes % 16 != 0) return fail(RTDBS_EINVAL, "device copy needs 16-byte multiples");
+    HIPCHK(launch_copy16(src_dev, dst_dev, bytes / 16, 0));
+    HIPCHK(hipDeviceSynchronize());
+    return RTDBS_OK;
+}
+
 extern "C" int rtdbs_device_synchronize(void) {
     HIPCHK(hipDeviceSynchronize());
     return RTDBS_OK;

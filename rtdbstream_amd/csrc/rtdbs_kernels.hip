@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "rtdbs_dev.h"
 
 namespace rtdbs {
@@ -152,13 +154,7 @@ __device__ void block_stream_prefix(int n, long long *pre, long long *tmp4, F va
 // L2 atomics, so stale reads cost extra steps, never correctness. Cacheable
 // loads keep the hot root entries of large clusters from funnelling every walk
 // into one L2 line.
-__device__ __forceinline__ int uf_load(const int32_t *p) {
-#ifdef RTDBS_UF_COHERENT_LOADS
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#else
-    return *p;
-#endif
-}
+__device__ __forceinline__ int uf_load(const int32_t *p) { return *p; }
 
 // Union-find over ring slots, parent[x] <= x for every slot, so the root of a
 // component is its minimum slot - the reference's canonical cluster id (it
@@ -175,13 +171,7 @@ __device__ __forceinline__ int uf_find(int32_t *parent, int x) {
     int p = uf_load(parent + x);
     while (p != x) {
         const int gp = uf_load(parent + p);
-#ifndef RTDBS_UF_NO_HALVING
-#ifdef RTDBS_UF_HALVE_STORE
-        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#else
         if (gp != p) atomicMin(parent + x, gp);
-#endif
-#endif
         x = p;
         p = gp;
     }
@@ -198,10 +188,6 @@ __device__ __forceinline__ void uf_unite(int32_t *parent, int a, int b) {
             a = b;
             b = t;
         }
-#ifdef RTDBS_TIMING_PLAIN_HOOK
-        parent[b] = a;  // racy: timing experiments only
-        return;
-#endif
         const int old = atomicMin(parent + b, a);
         if (old == b) return;
         b = old;
@@ -964,9 +950,7 @@ __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
 constexpr int kEdgeBuf = 512;
 
 __device__ __forceinline__ void flush_edges(const StreamDev &S, int2 *ebuf, int &ecnt) {
-#ifndef RTDBS_TIMING_NO_UNITE
     for (int i = lane_id(); i < ecnt; i += 64) uf_unite(S.parent, ebuf[i].x, ebuf[i].y);
-#endif
     ecnt = 0;
 }
 
@@ -1234,6 +1218,22 @@ __global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
          i += (size_t)gridDim.x * kTileThreads)
         S.htab[i] = 0ull;
     if (blockIdx.x == 0 && threadIdx.x == 0) S.mut->epoch = 1u;
+}
+
+// Plain device copy of 16-byte words: a kernel of known HBM traffic (reads and
+// writes `n16` * 16 bytes) used to scale the profiler's byte counters.
+__global__ __launch_bounds__(kTileThreads) void k_copy16(const int4 *__restrict__ src, int4 *__restrict__ dst,
+                                                         size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * kTileThreads + threadIdx.x; i < n16; i += (size_t)gridDim.x * kTileThreads)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void *src, void *dst, size_t n16, hipStream_t st) {
+    if (n16 == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n16 + kTileThreads - 1) / kTileThreads, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(kTileThreads), 0, st, (const int4 *)src, (int4 *)dst,
+                       n16);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
