@@ -8,7 +8,10 @@ All inputs are resident in HBM before the timed region.
 
 Also reported on the same JSON line:
   roofline      - the range-count kernel: algorithmic bytes per launch
-                  (rtdbstream_amd/roofline.py) / its average HIP-event duration
+                  (rtdbstream_amd/roofline.py) / its average duration from HIP
+                  events recorded by its own dispatch; `traffic` is the HBM
+                  bytes per launch from the rocprofv3 counter passes summarised
+                  in profiles/k_count_<workload>_traffic.md
   cpu_baseline  - the CPU port (oracle, OpenMP) on a bounded sample of the same
                   slides, whose labels are also compared with the GPU's
   batched       - the 64-stream batch: one launch per phase, HIP graph, vs the
@@ -87,6 +90,20 @@ def plain_leg(R, WL, name, steps, warmup):
                 eps=eps, min_pts=mp, warmup=warmup, steps=steps)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the range-count kernel on this workload, from
+    the two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by
+    what it reads for a copy kernel of known size) summarised under profiles/.
+    None when no such summary exists for the workload."""
+    path = os.path.join(ROOT, "profiles", "k_count_%s_traffic.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def roofline(WL, res, sample=8):
     from rtdbstream_amd import roofline as RF
 
@@ -106,7 +123,7 @@ def roofline(WL, res, sample=8):
     avg_s = ms / max(1, n) / 1e3
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(res["cfg_name"]),
             "kernel": "k_count", "algorithmic_bytes_per_launch": int(B),
             "avg_launch_us": round(avg_s * 1e6, 2)}
 

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "rtdbs_dev.h"
 
 namespace rtdbs {
@@ -1251,6 +1253,13 @@ static inline hipError_t mark(hipEvent_t *ev, uint32_t mask, int idx, hipStream_
     return hipEventRecord(ev[idx], st);
 }
 
+template <int LPQ>
+static hipError_t launch_count(const BatchArgs &a, int chunks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    if (e0) hipExtLaunchKernelGGL(k_count<LPQ>, dim3(chunks * LPQ), dim3(64), 0, st, e0, e1, 0, a);
+    else hipLaunchKernelGGL(k_count<LPQ>, dim3(chunks * LPQ), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t *ev,
                         uint32_t phase_mask) {
     hipError_t e;
@@ -1275,11 +1284,17 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     PH_BEGIN(kPhRows);
     RTDBS_LAUNCH(k_rows, sh.persist_blocks, kTileThreads, a);
     PH_END(kPhRows);
-    PH_BEGIN(kPhCount);
-    if (sh.lpq >= 4) RTDBS_LAUNCH(k_count<4>, sh.chunks * 4, 64, a);
-    else if (sh.lpq == 2) RTDBS_LAUNCH(k_count<2>, sh.chunks * 2, 64, a);
-    else RTDBS_LAUNCH(k_count<1>, sh.chunks, 64, a);
-    PH_END(kPhCount);
+    {
+        // The range-count kernel is timed by its own dispatch: the events take
+        // the kernel's begin and end timestamps, with no marker packets of
+        // their own between it and its neighbours.
+        const bool timed = ev && ((phase_mask >> kPhCount) & 1u);
+        const hipEvent_t e0 = timed ? ev[2 * kPhCount] : nullptr, e1 = timed ? ev[2 * kPhCount + 1] : nullptr;
+        if (sh.lpq >= 4) e = launch_count<4>(a, sh.chunks, st, e0, e1);
+        else if (sh.lpq == 2) e = launch_count<2>(a, sh.chunks, st, e0, e1);
+        else e = launch_count<1>(a, sh.chunks, st, e0, e1);
+        if (e != hipSuccess) return e;
+    }
     PH_BEGIN(kPhUnion);
     RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
     RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);

@@ -141,7 +141,7 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
         return RTDBS_OK;
     }
     double best = 1e300;
-    int bk = 2, bl = 0;
+    int bk = 2, bl = 1;
     for (int kk = 1; kk <= kMaxK; kk++) {
         rtdbs_params p = p0;
         p.cell_div = kk;
@@ -155,6 +155,9 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
             bk = kk;
         }
     }
+    // the automatic lanes-per-query choice of the k trials is one of these
+    // three, so the result always names an explicit setting
+    best = 1e300;
     for (int ll : {1, 2, 4}) {
         rtdbs_params p = p0;
         p.cell_div = bk;
