@@ -20,6 +20,7 @@ constexpr int kMaxRows = 49;       // (2k+1)^(dim-1) for k <= 3, dim <= 3
 constexpr int kKeyBits = 40;       // cell key bits; slot bits = 64 - 40 at most 24
 constexpr int kMaxSlotBits = 24;
 constexpr uint32_t kEpochMax = (1u << 24) - 1;  // hash epochs cycle 1..kEpochMax
+constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
 constexpr int kNumCounters = 8;
 

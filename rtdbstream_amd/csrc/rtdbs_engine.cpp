@@ -571,7 +571,8 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
-    b->shape.cell_blocks = (int)std::max(1ll, std::min((long long)device_cus() * 32, cells_total));
+    b->shape.cell_blocks =
+        (int)std::max(1ll, std::min((long long)device_cus() * 32, (cells_total + kUGCells - 1) / kUGCells));
     // lanes per query: enough threads to cover the device several times over,
     // unless a single stream asks for a tuned setting
     const long long lanes_wanted = (long long)device_cus() * 32 * 64 * 4;

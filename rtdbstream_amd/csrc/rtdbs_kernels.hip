@@ -945,188 +945,408 @@ __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
     }
 }
 
-// Per-wave edge buffer in LDS: touching pairs are collected while the wave
-// walks its cells and united lane-parallel when the buffer fills (and at the
-// end), so the latency of the union-find walks overlaps across cells instead
-// of adding up cell after cell. Appending needs no atomics.
-constexpr int kEdgeBuf = 512;
-
-__device__ __forceinline__ void flush_edges(const StreamDev &S, int2 *ebuf, int &ecnt) {
-    for (int i = lane_id(); i < ecnt; i += 64) uf_unite(S.parent, ebuf[i].x, ebuf[i].y);
-    ecnt = 0;
-}
-
-__device__ __forceinline__ void unite_if(const StreamDev &S, int2 *ebuf, int &ecnt, bool want, int u, int v) {
-    const uint64_t m = __ballot(want);
-    if (m == 0ull) return;
-    const int n = __popcll(m);
-    if (ecnt + n > kEdgeBuf) flush_edges(S, ebuf, ecnt);
-    if (want) ebuf[ecnt + __popcll(m & ((1ull << lane_id()) - 1ull))] = make_int2(u, v);
-    ecnt += n;
-}
-
 // Cluster merge, cell level, for safe cells: all cores of a safe cell belong
 // to one cluster (its points are within eps of each other), so two safe cells
 // join as soon as one core pair is within eps - the analogue of the
 // reference's dense-cell spheres, where a ray unites with the first point of
-// the cell within eps and stops (dbscan.cu:378-393). Each lane takes one
-// stencil cell B < A, so each unordered pair is tested once, and unites the
-// two cells' representatives when they touch. A's points sit in LDS; a small
-// B is held in the lane's registers, so the pair test is arithmetic only.
-constexpr int kUnionA = 256;  // LDS capacity for the query cell
-constexpr int kUnionB = 8;    // register capacity for a candidate cell
+// the cell within eps and stops (dbscan.cu:378-393). Only stencil cells
+// B < A are tested, so each unordered pair of cells is tested once.
+//
+// A wave serves kUGCells query cells per round. Each query cell's cores are
+// staged in LDS; the stencil rows of all the round's cells are spread over the
+// lanes, their candidate cells are concatenated into one list, and each lane
+// tests one (query cell, candidate) pair at a time. Touching pairs go to an
+// LDS edge list that the lanes unite together at the end of the round, so a
+// round costs a few dependent memory round trips for all its cells. Pairs
+// needing more than kUGWork point tests are queued and tested by the whole wave;
+// query cells with more than kUGCap cores are served by the whole wave
+// (union_cell_wave).
+constexpr int kUG = 64 / kUGCells;            // lanes staging one query cell
+constexpr int kUGCap = 64;                    // cores of a query cell staged in LDS
+constexpr int kUGWork = 256;                  // larger pair tests (cores x points): whole wave
+constexpr int kUGQueue = 128;                 // queued (query cell, candidate) pairs
+constexpr int kUGRows = kUGCells * kMaxRows;  // stencil rows of one round
+constexpr int kUGEdges = 256;                 // buffered touching pairs
+constexpr int kUnionA = kUGCells * kUGCap;    // LDS capacity for a large query cell
+constexpr int kUnionB = 8;                    // register capacity for a candidate cell
 
-__global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
-    __shared__ float lx[kUnionA], ly[kUnionA], lz[kUnionA];
-    __shared__ uint8_t lc[kUnionA];
-    __shared__ int2 ebuf[kEdgeBuf];
-    const int lane = lane_id();
-    const int total = a.cell_pre[a.n];
-    int ecnt = 0;
-    int es = -1;  // stream of the buffered edges
-    for (int gc = blockIdx.x; gc < total; gc += gridDim.x) {
-        const CellCtx x = cell_of(a, gc);
-        const StreamDev &S = a.s[x.s];
-        if (x.s != es) {
-            if (es >= 0) {
-                __syncthreads();
-                flush_edges(a.s[es], ebuf, ecnt);
-            }
-            es = x.s;
-        }
-        const int A = x.c;
-        const uint8_t cfA = S.cflag[A];
-        if (!(cfA & kCellSafe)) continue;
-        const int fcA = S.fcore[A];
-        if (fcA == kNoCore) continue;
-        const int dim = S.dim;
-        const float e2 = S.eps2;
-        // stencil cells B < A, row by row: [pcell[rb], min(pcell[re-1], A-1)]
-        int clo = 0, cnt = 0;
-        if (lane < S.R) {
-            const int2 r = S.rows[(size_t)A * S.R + lane];
-            if (r.x < r.y) {
-                clo = S.pcell[r.x];
-                const int chi = min(S.pcell[r.y - 1], A - 1);
-                cnt = max(0, chi - clo + 1);
-            }
-        }
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        const int pre = incl - cnt;
-        const int TC = __shfl(incl, 63);
-        if (TC == 0) continue;
-        const int csA = S.cstart[A], ceA = S.cstart[A + 1], nA = ceA - csA;
-        const bool inl = nA <= kUnionA;
-        const int sA = slot_of(S, fcA);
-        __syncthreads();  // the previous cell's readers are done with the LDS
-        if (inl) {
-            for (int i = lane; i < nA; i += 64) {
-                lx[i] = S.sx[csA + i];
-                ly[i] = dim > 1 ? S.sy[csA + i] : 0.f;
-                lz[i] = dim > 2 ? S.sz[csA + i] : 0.f;
-                lc[i] = (cfA & kCellDense) ? (uint8_t)1 : (uint8_t)(S.flags[csA + i] & kFlagCore);
-            }
-        }
+struct UnionLds {
+    float x[kUGCells * kUGCap], y[kUGCells * kUGCap], z[kUGCells * kUGCap];
+    uint8_t lc[kUnionA];       // whole-wave path: core flags of the query cell
+    int ncore[kUGCells];       // cores staged for the group's cell
+    int rep[kUGCells];         // representative slot of the group's cell
+    int sidx[kUGCells];        // stream of the group's cell
+    int cell[kUGCells];        // the group's cell (stream-local)
+    int mode[kUGCells];        // 0 idle, 1 staged, 2 whole-wave path
+    int rowbase[kUGCells + 1]; // first row of each group in the round's row list
+    int rowpre[kUGRows + 1];   // exclusive prefix of candidate counts
+    int rowclo[kUGRows];       // first candidate cell of the row
+    int2 q[kUGQueue];          // (group, candidate cell)
+    int2 edge[kUGEdges];       // (group, representative slot of the candidate)
+    int nq;
+};
+
+__device__ __forceinline__ void flush_edges(const BatchArgs &a, UnionLds &L, int &ne) {
+    for (int i = lane_id(); i < ne; i += 64) {
+        const int2 e = L.edge[i];
+        uf_unite(a.s[L.sidx[e.x]].parent, L.rep[e.x], e.y);
+    }
+    ne = 0;
+}
+
+// Appends the touching pairs of the wave (lanes with want set) to the edge
+// list, in lane order; all lanes call it together.
+__device__ __forceinline__ void push_edges(const BatchArgs &a, UnionLds &L, int &ne, bool want, int g, int v) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return;
+    const int n = __popcll(m);
+    if (ne + n > kUGEdges) {
         __syncthreads();
-        for (int jb = 0; jb < TC; jb += 64) {
-            const int j = jb + lane;
-            const int jj = j < TC ? j : TC - 1;
-            const int r = row_of(pre, S.R, jj);
-            const int B = __shfl(clo, r) + (jj - __shfl(pre, r));
-            bool big = false;
-            bool touch = false;
-            int fcB = kNoCore;
-            if (j < TC) {
-                const uint8_t cfB = S.cflag[B];
-                fcB = S.fcore[B];
-                if ((cfB & kCellSafe) && fcB != kNoCore) {
-                    const int csB = S.cstart[B], nB = S.cstart[B + 1] - csB;
-                    if (inl && nB <= kUnionB) {
-                        // small cell: its points in registers, A's cores from LDS
-                        float bx[kUnionB], by[kUnionB], bz[kUnionB];
-                        bool bc[kUnionB];
-#pragma unroll
-                        for (int m = 0; m < kUnionB; m++) {
-                            bc[m] = false;
-                            bx[m] = by[m] = bz[m] = 0.f;
-                            if (m < nB) {
-                                bc[m] = (cfB & kCellDense) || (S.flags[csB + m] & kFlagCore);
-                                bx[m] = S.sx[csB + m];
-                                if (dim > 1) by[m] = S.sy[csB + m];
-                                if (dim > 2) bz[m] = S.sz[csB + m];
-                            }
-                        }
-                        for (int i = 0; i < nA && !touch; i++) {
-                            if (!lc[i]) continue;
-                            const float qx = lx[i], qy = ly[i], qz = lz[i];
-#pragma unroll
-                            for (int m = 0; m < kUnionB; m++)
-                                touch = touch || (bc[m] && sqd(dim, qx, qy, qz, bx[m], by[m], bz[m]) < e2);
-                        }
-                    } else {
-                        big = true;
-                    }
-                }
-            }
-            __syncthreads();
-            unite_if(S, ebuf, ecnt, touch, sA, touch ? slot_of(S, fcB) : 0);
-            // large candidate cells of this batch: one at a time, lanes over
-            // B's points, loop over A's cores, stop at the first pair within eps
-            uint64_t bigm = __ballot(big);
-            while (bigm) {
-                const int l = __ffsll((unsigned long long)bigm) - 1;
-                bigm &= bigm - 1;
-                const int Bl = lane_i(B, l);
-                const uint8_t cfB = S.cflag[Bl];
-                const int fcB = S.fcore[Bl];
-                const int csB = S.cstart[Bl], ceB = S.cstart[Bl + 1];
-                bool touch = false;
-                {
-                    for (int b0 = csB; b0 < ceB && !touch; b0 += 64) {
-                        const int pb = b0 + lane;
-                        bool vb = pb < ceB;
-                        float bx = 0.f, by = 0.f, bz = 0.f;
-                        if (vb) {
-                            vb = (cfB & kCellDense) || (S.flags[pb] & kFlagCore);
-                            bx = S.sx[pb];
-                            if (dim > 1) by = S.sy[pb];
-                            if (dim > 2) bz = S.sz[pb];
-                        }
-                        for (int i = 0; i < nA; i++) {
-                            float qx, qy, qz;
-                            bool qc;
-                            if (inl) {
-                                qc = lc[i];
-                                qx = lx[i];
-                                qy = ly[i];
-                                qz = lz[i];
-                            } else {
-                                qc = (cfA & kCellDense) || (S.flags[csA + i] & kFlagCore);
-                                qx = S.sx[csA + i];
-                                qy = dim > 1 ? S.sy[csA + i] : 0.f;
-                                qz = dim > 2 ? S.sz[csA + i] : 0.f;
-                            }
-                            if (!qc) continue;
-                            if (__ballot(vb && sqd(dim, qx, qy, qz, bx, by, bz) < e2) != 0ull) {
-                                touch = true;
-                                break;
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-                unite_if(S, ebuf, ecnt, touch && lane == 0, sA, slot_of(S, fcB));
+        flush_edges(a, L, ne);
+        __syncthreads();
+    }
+    if (want) L.edge[ne + __popcll(m & ((1ull << lane_id()) - 1ull))] = make_int2(g, v);
+    ne += n;
+}
+
+// Whole-wave test of one (query cell, large candidate cell B) pair: lanes
+// over B's points, a loop over the query cell's staged cores, stop at the
+// first pair within eps.
+__device__ void union_pair_wave(const StreamDev &S, const float *qx, const float *qy, const float *qz, int ncore,
+                                int rep, int B) {
+    const int lane = lane_id(), dim = S.dim;
+    const float e2 = S.eps2;
+    const uint8_t cfB = S.cflag[B];
+    const int fcB = S.fcore[B];
+    const int csB = S.cstart[B], ceB = S.cstart[B + 1];
+    bool touch = false;
+    for (int b0 = csB; b0 < ceB && !touch; b0 += 64) {
+        const int pb = b0 + lane;
+        bool vb = pb < ceB;
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        if (vb) {
+            vb = (cfB & kCellDense) || (S.flags[pb] & kFlagCore);
+            bx = S.sx[pb];
+            if (dim > 1) by = S.sy[pb];
+            if (dim > 2) bz = S.sz[pb];
+        }
+        for (int i = 0; i < ncore; i++) {
+            if (__ballot(vb && sqd(dim, qx[i], qy[i], qz[i], bx, by, bz) < e2) != 0ull) {
+                touch = true;
+                break;
             }
         }
     }
-    if (es >= 0) {
+    if (touch && lane == 0) uf_unite(S.parent, rep, slot_of(S, fcB));
+}
+
+// Whole-wave merge of one large safe query cell A with its stencil cells
+// B < A: lanes take candidate cells; A's points sit in LDS when they fit,
+// and a small B is held in the lane's registers.
+__device__ void union_cell_wave(const StreamDev &S, int A, UnionLds &L) {
+    const int lane = lane_id();
+    const uint8_t cfA = S.cflag[A];
+    const int fcA = S.fcore[A];
+    const int dim = S.dim;
+    const float e2 = S.eps2;
+    float *lx = L.x, *ly = L.y, *lz = L.z;
+    // stencil cells B < A, row by row: [pcell[rb], min(pcell[re-1], A-1)]
+    int clo = 0, cnt = 0;
+    if (lane < S.R) {
+        const int2 r = S.rows[(size_t)A * S.R + lane];
+        if (r.x < r.y) {
+            clo = S.pcell[r.x];
+            const int chi = min(S.pcell[r.y - 1], A - 1);
+            cnt = max(0, chi - clo + 1);
+        }
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int pre = incl - cnt;
+    const int TC = __shfl(incl, 63);
+    if (TC == 0) return;
+    const int csA = S.cstart[A], ceA = S.cstart[A + 1], nA = ceA - csA;
+    const bool inl = nA <= kUnionA;
+    const int sA = slot_of(S, fcA);
+    __syncthreads();
+    if (inl) {
+        for (int i = lane; i < nA; i += 64) {
+            lx[i] = S.sx[csA + i];
+            ly[i] = dim > 1 ? S.sy[csA + i] : 0.f;
+            lz[i] = dim > 2 ? S.sz[csA + i] : 0.f;
+            L.lc[i] = (cfA & kCellDense) ? (uint8_t)1 : (uint8_t)(S.flags[csA + i] & kFlagCore);
+        }
+    }
+    __syncthreads();
+    for (int jb = 0; jb < TC; jb += 64) {
+        const int j = jb + lane;
+        const int jj = j < TC ? j : TC - 1;
+        const int r = row_of(pre, S.R, jj);
+        const int B = __shfl(clo, r) + (jj - __shfl(pre, r));
+        bool big = false;
+        bool touch = false;
+        int fcB = kNoCore;
+        if (j < TC) {
+            const uint8_t cfB = S.cflag[B];
+            fcB = S.fcore[B];
+            if ((cfB & kCellSafe) && fcB != kNoCore) {
+                const int csB = S.cstart[B], nB = S.cstart[B + 1] - csB;
+                if (inl && nB <= kUnionB) {
+                    float bx[kUnionB], by[kUnionB], bz[kUnionB];
+                    bool bc[kUnionB];
+#pragma unroll
+                    for (int m = 0; m < kUnionB; m++) {
+                        bc[m] = false;
+                        bx[m] = by[m] = bz[m] = 0.f;
+                        if (m < nB) {
+                            bc[m] = (cfB & kCellDense) || (S.flags[csB + m] & kFlagCore);
+                            bx[m] = S.sx[csB + m];
+                            if (dim > 1) by[m] = S.sy[csB + m];
+                            if (dim > 2) bz[m] = S.sz[csB + m];
+                        }
+                    }
+                    for (int i = 0; i < nA && !touch; i++) {
+                        if (!L.lc[i]) continue;
+                        const float qx = lx[i], qy = ly[i], qz = lz[i];
+#pragma unroll
+                        for (int m = 0; m < kUnionB; m++)
+                            touch = touch || (bc[m] && sqd(dim, qx, qy, qz, bx[m], by[m], bz[m]) < e2);
+                    }
+                } else {
+                    big = true;
+                }
+            }
+        }
+        if (touch) uf_unite(S.parent, sA, slot_of(S, fcB));
+        // large candidate cells of this batch: one at a time, lanes over B's
+        // points, a loop over A's cores, stop at the first pair within eps
+        uint64_t bigm = __ballot(big);
+        while (bigm) {
+            const int l = __ffsll((unsigned long long)bigm) - 1;
+            bigm &= bigm - 1;
+            const int Bl = lane_i(B, l);
+            const uint8_t cfB = S.cflag[Bl];
+            const int fcBl = S.fcore[Bl];
+            const int csB = S.cstart[Bl], ceB = S.cstart[Bl + 1];
+            bool t = false;
+            for (int b0 = csB; b0 < ceB && !t; b0 += 64) {
+                const int pb = b0 + lane;
+                bool vb = pb < ceB;
+                float bx = 0.f, by = 0.f, bz = 0.f;
+                if (vb) {
+                    vb = (cfB & kCellDense) || (S.flags[pb] & kFlagCore);
+                    bx = S.sx[pb];
+                    if (dim > 1) by = S.sy[pb];
+                    if (dim > 2) bz = S.sz[pb];
+                }
+                for (int i = 0; i < nA; i++) {
+                    float qx, qy, qz;
+                    bool qc;
+                    if (inl) {
+                        qc = L.lc[i];
+                        qx = lx[i];
+                        qy = ly[i];
+                        qz = lz[i];
+                    } else {
+                        qc = (cfA & kCellDense) || (S.flags[csA + i] & kFlagCore);
+                        qx = S.sx[csA + i];
+                        qy = dim > 1 ? S.sy[csA + i] : 0.f;
+                        qz = dim > 2 ? S.sz[csA + i] : 0.f;
+                    }
+                    if (!qc) continue;
+                    if (__ballot(vb && sqd(dim, qx, qy, qz, bx, by, bz) < e2) != 0ull) {
+                        t = true;
+                        break;
+                    }
+                }
+            }
+            if (t && lane == 0) uf_unite(S.parent, sA, slot_of(S, fcBl));
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
+    __shared__ UnionLds L;
+    const int lane = lane_id(), grp = lane / kUG, sub = lane % kUG;
+    const uint64_t gmask = ((1ull << kUG) - 1ull) << (grp * kUG);
+    const int total = a.cell_pre[a.n];
+    // Round r of wave b serves cells b + (r * kUGCells + g) * gridDim.x: the
+    // cells of a round are far apart, so dense regions (long pair tests) are
+    // spread over many waves instead of falling to a few.
+    const int G = gridDim.x;
+    for (int base = 0; base + (int)blockIdx.x < total; base += G * kUGCells) {
+        // 1. stage each group's query cell: its cores, compacted, in LDS
+        const int gc = (int)blockIdx.x + base + grp * G;
+        int mode = 0, ncore = 0, A = 0, sidx = 0, rep = 0, R = 0;
+        if (gc < total) {
+            const CellCtx x = cell_of(a, gc);
+            sidx = x.s;
+            A = x.c;
+            const StreamDev &S = a.s[sidx];
+            const uint8_t cfA = S.cflag[A];
+            const int fcA = S.fcore[A];
+            if ((cfA & kCellSafe) && fcA != kNoCore) {
+                const int csA = S.cstart[A], nA = S.cstart[A + 1] - csA;
+                const bool denseA = cfA & kCellDense;
+                const int dim = S.dim;
+                float *gx = L.x + grp * kUGCap, *gy = L.y + grp * kUGCap, *gz = L.z + grp * kUGCap;
+                rep = slot_of(S, fcA);
+                for (int i0 = 0; i0 < nA; i0 += kUG) {
+                    const int i = i0 + sub;
+                    bool c = false;
+                    float px = 0.f, py = 0.f, pz = 0.f;
+                    if (i < nA) {
+                        c = denseA || (S.flags[csA + i] & kFlagCore);
+                        px = S.sx[csA + i];
+                        if (dim > 1) py = S.sy[csA + i];
+                        if (dim > 2) pz = S.sz[csA + i];
+                    }
+                    const uint64_t m = __ballot(c) & gmask;
+                    const int pos = ncore + __popcll(m & ((1ull << lane) - 1ull));
+                    if (c && pos < kUGCap) {
+                        gx[pos] = px;
+                        gy[pos] = py;
+                        gz[pos] = pz;
+                    }
+                    ncore += __popcll(m);
+                }
+                mode = ncore <= kUGCap ? 1 : 2;
+                R = mode == 1 ? S.R : 0;
+            }
+        }
+        if (sub == 0) {
+            L.mode[grp] = mode;
+            L.ncore[grp] = ncore;
+            L.rep[grp] = rep;
+            L.sidx[grp] = sidx;
+            L.cell[grp] = A;
+        }
+        // rows of group g occupy [rbase[g], rbase[g+1]) of the round's row list
+        int rbase[kUGCells + 1];
+        rbase[0] = 0;
+#pragma unroll
+        for (int g = 0; g < kUGCells; g++) rbase[g + 1] = rbase[g] + __shfl(R, g * kUG);
+        if (lane <= kUGCells) {
+            int v = 0;
+#pragma unroll
+            for (int g = 0; g <= kUGCells; g++) v = lane == g ? rbase[g] : v;
+            L.rowbase[lane] = v;
+        }
+        const int NR = rbase[kUGCells];
         __syncthreads();
-        flush_edges(a.s[es], ebuf, ecnt);
+        // 2. rows of all groups over the lanes: candidate cells B < A per row
+        int run = 0;
+        for (int t0 = 0; t0 < NR; t0 += 64) {
+            const int t = t0 + lane;
+            int cnt = 0, clo = 0;
+            if (t < NR) {
+                int g = 0;
+#pragma unroll
+                for (int h = 1; h < kUGCells; h++) g += t >= rbase[h];
+                const StreamDev &S = a.s[L.sidx[g]];
+                const int Ag = L.cell[g];
+                const int2 rr = S.rows[(size_t)Ag * S.R + (t - L.rowbase[g])];
+                if (rr.x < rr.y) {
+                    clo = S.pcell[rr.x];
+                    const int chi = min(S.pcell[rr.y - 1], Ag - 1);
+                    cnt = max(0, chi - clo + 1);
+                }
+                L.rowclo[t] = clo;
+            }
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            if (t < NR) L.rowpre[t] = run + incl - cnt;
+            run += __shfl(incl, 63);
+        }
+        const int TC = run;
+        if (lane == 0) {
+            L.rowpre[NR] = TC;
+            L.nq = 0;
+        }
+        __syncthreads();
+        // 3. (query cell, candidate) pairs over the lanes
+        int ne = 0;
+        for (int jb = 0; jb < TC; jb += 64) {
+            const int j = jb + lane;
+            bool touch = false;
+            int g = 0, vrep = 0;
+            if (j < TC) {
+                int lo = 0, hi = NR;  // last row t with rowpre[t] <= j
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.rowpre[mid] <= j) lo = mid;
+                    else hi = mid;
+                }
+                const int t = lo;
+#pragma unroll
+                for (int h = 1; h < kUGCells; h++) g += t >= rbase[h];
+                const int B = L.rowclo[t] + (j - L.rowpre[t]);
+                const StreamDev &S = a.s[L.sidx[g]];
+                const uint8_t cfB = S.cflag[B];
+                const int fcB = S.fcore[B];
+                if ((cfB & kCellSafe) && fcB != kNoCore) {
+                    const int csB = S.cstart[B], nB = S.cstart[B + 1] - csB;
+                    bool queued = false;
+                    if (L.ncore[g] * nB > kUGWork) {
+                        const int e = atomicAdd(&L.nq, 1);
+                        if (e < kUGQueue) {
+                            L.q[e] = make_int2(g, B);
+                            queued = true;
+                        }
+                    }
+                    if (!queued) {
+                        const int dim = S.dim, nc = L.ncore[g];
+                        const float e2 = S.eps2;
+                        const float *qx = L.x + g * kUGCap, *qy = L.y + g * kUGCap, *qz = L.z + g * kUGCap;
+                        const bool denseB = cfB & kCellDense;
+                        for (int b0 = 0; b0 < nB && !touch; b0 += 4) {
+                            float bx[4], by[4], bz[4];
+                            bool bc[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int jj = csB + min(b0 + u, nB - 1);
+                                bc[u] = (b0 + u < nB) && (denseB || (S.flags[jj] & kFlagCore));
+                                bx[u] = S.sx[jj];
+                                by[u] = dim > 1 ? S.sy[jj] : 0.f;
+                                bz[u] = dim > 2 ? S.sz[jj] : 0.f;
+                            }
+                            for (int i = 0; i < nc && !touch; i++) {
+                                const float ax = qx[i], ay = qy[i], az = qz[i];
+#pragma unroll
+                                for (int u = 0; u < 4; u++)
+                                    touch = touch || (bc[u] && sqd(dim, ax, ay, az, bx[u], by[u], bz[u]) < e2);
+                            }
+                        }
+                        if (touch) vrep = slot_of(S, fcB);
+                    }
+                }
+            }
+            push_edges(a, L, ne, touch, g, vrep);
+        }
+        __syncthreads();
+        flush_edges(a, L, ne);
+        __syncthreads();
+        // 4. queued large candidate cells, then large query cells: whole wave
+        const int nq = min(L.nq, kUGQueue);
+        for (int e = 0; e < nq; e++) {
+            const int2 it = L.q[e];
+            const int g = it.x;
+            union_pair_wave(a.s[L.sidx[g]], L.x + g * kUGCap, L.y + g * kUGCap, L.z + g * kUGCap, L.ncore[g],
+                            L.rep[g], it.y);
+        }
+        for (int g = 0; g < kUGCells; g++) {
+            if (L.mode[g] != 2) continue;
+            union_cell_wave(a.s[L.sidx[g]], L.cell[g], L);
+        }
+        __syncthreads();  // the next round overwrites the LDS
     }
 }
 
