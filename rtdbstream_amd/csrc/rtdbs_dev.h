@@ -11,7 +11,8 @@
 
 namespace rtdbs {
 
-constexpr int kTile = 2048;        // positions per tile: stride sort, merge, compaction
+constexpr int kTile = 2048;        // positions per sort tile (stride, first window)
+constexpr int kWTile = 512;        // positions per window tile: expiry, merge, cell table
 constexpr int kTileThreads = 256;  // threads per tile block
 constexpr int kChunk = 64;         // positions per wave in the neighbourhood kernels
 constexpr int kCap = 512;          // candidates staged in LDS per wave
@@ -63,7 +64,7 @@ struct StreamDev {
     // compacted survivors of the previous window (W), also init ping-pong
     uint64_t *akey;
     float *ax, *ay, *az;
-    // stride sort ping-pong (S)
+    // stride: sorted tiles (keys, bkey0), then the sorted stride (B1)
     uint64_t *bkey0;
     float *bx0, *by0, *bz0;
     uint64_t *bkey1;
@@ -81,8 +82,8 @@ struct StreamDev {
     uint8_t *flags;   // W: kFlagCore | kFlagDense
     int32_t *parent;  // W: union-find over positions, root = minimum slot
     int32_t *canon;   // W: canonical id (min core slot) of core positions
-    int32_t *tile_a;  // ceil(W/kTile): survivors per tile
-    int32_t *tile_b;  // ceil(W/kTile): cell starts per tile
+    int32_t *tile_a;  // ceil(W/kWTile): survivors per tile
+    int32_t *tile_b;  // ceil(W/kWTile): cell starts per tile
 };
 
 struct BatchArgs {
@@ -92,7 +93,8 @@ struct BatchArgs {
     int32_t use_io1;
     int32_t n;
     const int32_t *pre_stile;  // n+1 prefix of ceil(S/kTile)
-    const int32_t *pre_wtile;  // n+1 prefix of ceil(W/kTile)
+    const int32_t *pre_wtile;  // n+1 prefix of ceil(W/kWTile)
+    const int32_t *pre_itile;  // n+1 prefix of ceil(W/kTile) (first-window sort)
     const int32_t *pre_chunk;  // n+1 prefix of ceil(W/kChunk)
     int32_t *counters;         // kNumCounters work counters
     int32_t *cell_pre;         // n+1 prefix of cells per stream (written by k_rows)
@@ -104,16 +106,15 @@ struct LaunchShape {
     int32_t stiles;   // total stride tiles
     int32_t wtiles;   // total window tiles
     int32_t chunks;   // total chunks
-    int32_t passes;   // stride merge passes
     int32_t persist_blocks;  // 256-thread blocks for the persistent row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
     int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
+    int32_t wave_blocks;     // grid cap of the persistent one-wave point kernels
 };
 
 enum Phase {
-    kPhStrideSort = 0,
-    kPhMerge,
-    kPhExpire,
+    kPhStrideSort = 0,  // stride tile sort + survivor counts
+    kPhMerge,           // stride rank merge + survivor compaction
     kPhMergeFinal,
     kPhCells,
     kPhRows,

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -127,7 +128,7 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
         return at;
     };
     const size_t W = (size_t)p.window, S = (size_t)p.stride, d = (size_t)p.dim;
-    const size_t ntile = (W + kTile - 1) / kTile;
+    const size_t ntile = (W + kWTile - 1) / kWTile;
     L.off_mut = take(sizeof(StreamMut));
     L.off_skey = take(W * 8);
     L.off_sx = take(W * 4);
@@ -478,7 +479,7 @@ static int batch_alloc(rtdbs_batch *b) {
     const int n = b->capacity;
     const size_t desc_b = ((size_t)n * sizeof(StreamDev) + 255) & ~size_t(255);
     const size_t io_b = ((size_t)n * sizeof(IO) + 255) & ~size_t(255);
-    const size_t pre_b = ((size_t)3 * (n + 1) * 4 + 255) & ~size_t(255);
+    const size_t pre_b = ((size_t)4 * (n + 1) * 4 + 255) & ~size_t(255);
     const size_t cnt_b = 256;
     const size_t cpre_b = ((size_t)(n + 1) * 4 + 255) & ~size_t(255);
     HIPCHK(hipMalloc((void **)&b->dmem, desc_b + io_b + pre_b + cnt_b + cpre_b));
@@ -539,25 +540,36 @@ void batch_free(rtdbs_batch *b) {
     delete b;
 }
 
+// One-wave workgroups per CU for the persistent cell-level merge
+// (RTDBS_WAVES_PER_CU overrides, 1..64).
+static int waves_per_cu() {
+    static int v = [] {
+        const char *e = std::getenv("RTDBS_WAVES_PER_CU");
+        int x = e ? std::atoi(e) : 0;
+        return (x >= 1 && x <= 64) ? x : 16;
+    }();
+    return v;
+}
+
 // Uploads descriptors and prefix tables; drops a captured graph. Called when
 // the batch is built and whenever a member stream was re-initialised.
 int refresh_batch(rtdbs_batch *b) {
     const int n = b->n;
     if (b->hs) HIPCHK(hipStreamSynchronize(b->hs));
     std::vector<StreamDev> desc(n);
-    std::vector<int32_t> pre(3 * (n + 1), 0);
-    int passes = 0;
+    std::vector<int32_t> pre(4 * (n + 1), 0);
     long long rows_total = 0, cells_total = 0;
     for (int i = 0; i < n; i++) {
         rtdbs_stream *s = b->streams[i];
         desc[i] = s->dev;
         const int st = (s->p.stride + kTile - 1) / kTile;
-        const int wt = (s->p.window + kTile - 1) / kTile;
+        const int wt = (s->p.window + kWTile - 1) / kWTile;
         const int ch = (s->p.window + kChunk - 1) / kChunk;
+        const int it = (s->p.window + kTile - 1) / kTile;
         pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + st;
         pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + wt;
         pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + ch;
-        passes = std::max(passes, ceil_log2(st));
+        pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + it;
         rows_total += (long long)s->p.window * s->R;
         cells_total += s->p.window;
         b->gens[i] = s->gen;
@@ -567,12 +579,15 @@ int refresh_batch(rtdbs_batch *b) {
     b->shape.stiles = pre[0 * (n + 1) + n];
     b->shape.wtiles = pre[1 * (n + 1) + n];
     b->shape.chunks = pre[2 * (n + 1) + n];
-    b->shape.passes = passes;
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
-    b->shape.cell_blocks =
-        (int)std::max(1ll, std::min((long long)device_cus() * 32, (cells_total + kUGCells - 1) / kUGCells));
+    // One-wave blocks: the point kernels take one block per virtual block (a
+    // bounded persistent grid measured slower, the waves' load chains being
+    // the limit, not their dispatch); the cell-level merge is capped.
+    b->shape.wave_blocks = 1 << 30;
+    b->shape.cell_blocks = (int)std::max(
+        1ll, std::min((long long)device_cus() * waves_per_cu(), (cells_total + kUGCells - 1) / kUGCells));
     // lanes per query: enough threads to cover the device several times over,
     // unless a single stream asks for a tuned setting
     const long long lanes_wanted = (long long)device_cus() * 32 * 64 * 4;
@@ -587,6 +602,7 @@ int refresh_batch(rtdbs_batch *b) {
     a.pre_stile = b->d_pre;
     a.pre_wtile = b->d_pre + (n + 1);
     a.pre_chunk = b->d_pre + 2 * (n + 1);
+    a.pre_itile = b->d_pre + 3 * (n + 1);
     a.counters = b->d_counters;
     a.cell_pre = b->d_cell_pre;
     if (b->gexec) {

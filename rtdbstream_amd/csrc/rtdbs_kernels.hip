@@ -26,7 +26,7 @@
 
 namespace rtdbs {
 
-const char *const kPhaseNames[kNumPhases] = {"stride_sort", "merge_passes", "expire", "merge_final",
+const char *const kPhaseNames[kNumPhases] = {"sort_expire", "rank_compact", "merge_final",
                                              "cells", "rows", "count", "union",
                                              "flatten", "border"};
 
@@ -288,72 +288,242 @@ __device__ __forceinline__ int hash_find(const StreamDev &S, uint64_t ck, uint32
 }
 
 // ---------------------------------------------------------------------------
-// sorting: LDS bitonic tile sort + merge-path merges
+// sorting: register bitonic tile sort, rank merge of the stride's tiles,
+// merge-path merges of the window
 
-__device__ __forceinline__ void bitonic_sort_tile(uint64_t *sk) {
-    constexpr int kPer = kTile / 2 / kTileThreads;  // compare-exchanges per thread and stage
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Sorts the kTile keys of a 256-thread block ascending; thread t holds
+// positions 8t..8t+7 in v. Bitonic network: exchanges between a thread's own
+// elements stay in registers, exchanges within a wave use lane shuffles, and
+// only the three stages whose partners sit in another wave go through LDS
+// (`lds`, kTile keys) with block barriers.
+__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *lds) {
+    const int tid = threadIdx.x;
+#pragma unroll
     for (int k = 2; k <= kTile; k <<= 1) {
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            int ia[kPer];
-            uint64_t va[kPer], vb[kPer];
+            if (j >= 8) {
+                const int m = j >> 3;  // partner thread = tid ^ m, same element index
+                const bool keep_min = ((tid & m) == 0) == (((tid * 8) & k) == 0);
+                if (j >= 512) {
+                    __syncthreads();
 #pragma unroll
-            for (int t = 0; t < kPer; t++) {
-                const int q = threadIdx.x + t * kTileThreads;
-                ia[t] = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                va[t] = sk[ia[t]];
-                vb[t] = sk[ia[t] + j];
-            }
+                    for (int e = 0; e < 8; e++) lds[tid * 8 + e] = v[e];
+                    __syncthreads();
 #pragma unroll
-            for (int t = 0; t < kPer; t++) {
-                const bool up = (ia[t] & k) == 0;
-                if ((va[t] > vb[t]) == up) {
-                    sk[ia[t]] = vb[t];
-                    sk[ia[t] + j] = va[t];
+                    for (int e = 0; e < 8; e++) {
+                        const uint64_t o = lds[(tid ^ m) * 8 + e];
+                        v[e] = keep_min ? (o < v[e] ? o : v[e]) : (o > v[e] ? o : v[e]);
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const uint64_t o = shfl_xor_u64(v[e], m);
+                        v[e] = keep_min ? (o < v[e] ? o : v[e]) : (o > v[e] ? o : v[e]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    if (e & j) continue;
+                    const bool up = (((tid * 8) + e) & k) == 0;
+                    const uint64_t x = v[e], y = v[e | j];
+                    const bool sw = (x > y) == up;
+                    v[e] = sw ? y : x;
+                    v[e | j] = sw ? x : y;
                 }
             }
-            __syncthreads();
         }
     }
 }
 
-// Cell keys of up to kTile points of a stride (or of the first window),
-// sorted by (cell, slot) in LDS and written with their coordinates.
-template <bool INIT>
-__global__ __launch_bounds__(kTileThreads) void k_stride_tiles(BatchArgs a) {
-    __shared__ uint64_t sk[kTile];
+// Loads the packed keys (cell << slot_bits | slot) of points [tile0, tile0 +
+// kTile) of `in` (len points, slots from slot0) into v, 8 per thread, and
+// sorts them; padding keys are ~0 and sort last.
+__device__ __forceinline__ void sort_tile_keys(const StreamDev &S, const float *in, int len, int tile0, int slot0,
+                                               uint64_t (&v)[8], uint64_t *lds) {
+    const int dim = S.dim;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int gi = tile0 + threadIdx.x * 8 + e;
+        v[e] = gi < len ? (cell_key_of(S, in + (size_t)gi * dim) << S.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
+    }
+    bitonic_sort_regs(v, lds);
+}
+
+// First window: keys and coordinates of each sorted tile (init ping-pong A).
+__global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
+    __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
-    if (g == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
-    const int32_t *pre = INIT ? a.pre_wtile : a.pre_stile;
-    const int s = find_stream(pre, a.n, g);
+    const int s = find_stream(a.pre_itile, a.n, g);
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int len = INIT ? S.W : S.S;
-    const int tile0 = (g - pre[s]) * kTile;
-    const int slot0 = INIT ? 0 : S.mut->u * S.S;
-    const int dim = S.dim;
-    for (int i = threadIdx.x; i < kTile; i += kTileThreads) {
-        const int gi = tile0 + i;
-        uint64_t key = ~0ull;
-        if (gi < len)
-            key = (cell_key_of(S, io.in + (size_t)gi * dim) << S.slot_bits) | (uint64_t)(slot0 + gi);
-        sk[i] = key;
-    }
-    __syncthreads();
-    bitonic_sort_tile(sk);
-    const int nvalid = min(kTile, len - tile0);
-    uint64_t *dk = INIT ? S.akey : S.bkey0;
-    float *dx = INIT ? S.ax : S.bx0;
-    float *dy = INIT ? S.ay : S.by0;
-    float *dz = INIT ? S.az : S.bz0;
+    const int tile0 = (g - a.pre_itile[s]) * kTile;
+    uint64_t v[8];
+    sort_tile_keys(S, io.in, S.W, tile0, 0, v, lds);
+    const int nvalid = min(kTile, S.W - tile0);
     const uint32_t m = slot_mask(S);
-    for (int i = threadIdx.x; i < nvalid; i += kTileThreads) {
-        const uint64_t key = sk[i];
-        const int gi = (int)((uint32_t)key & m) - slot0;
-        const float *p = io.in + (size_t)gi * dim;
-        dk[tile0 + i] = key;
-        dx[tile0 + i] = p[0];
-        if (dim > 1) dy[tile0 + i] = p[1];
-        if (dim > 2) dz[tile0 + i] = p[2];
+    const int dim = S.dim;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = threadIdx.x * 8 + e;
+        if (i < nvalid) {
+            const int gi = (int)((uint32_t)v[e] & m);
+            const float *p = io.in + (size_t)gi * dim;
+            S.akey[tile0 + i] = v[e];
+            S.ax[tile0 + i] = p[0];
+            if (dim > 1) S.ay[tile0 + i] = p[1];
+            if (dim > 2) S.az[tile0 + i] = p[2];
+        }
+    }
+}
+
+__device__ __forceinline__ bool survives(const StreamDev &S, uint64_t key, int lo_slot) {
+    const int slot = (int)((uint32_t)key & slot_mask(S));
+    return slot < lo_slot || slot >= lo_slot + S.S;
+}
+
+// Survivors per window tile (the reference's stable_partition,
+// thrust_helper.cu:37-39, split into count + compact).
+__device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds4) {
+    const int s = find_stream(a.pre_wtile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const int tile = g - a.pre_wtile[s];
+    const int p0 = tile * kWTile, p1 = min(p0 + kWTile, S.W);
+    const int lo_slot = S.mut->u * S.S;
+    int cnt = 0;
+    for (int p = p0 + threadIdx.x; p < p1; p += kTileThreads) cnt += survives(S, S.skey[p], lo_slot);
+    cnt = block_sum(cnt, lds4);
+    if (threadIdx.x == 0) S.tile_a[tile] = cnt;
+}
+
+// Launch 1 of a slide: blocks [0, stride tiles) sort the stride's tiles by
+// key (the reference's compute_cell_id + sortByCellIdAndOrder, aabb.cu:589-611,
+// thrust_helper.cu:21-29) into bkey0; the remaining blocks count the
+// survivors of each window tile.
+__global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
+    __shared__ uint64_t lds[kTile];
+    const int g = blockIdx.x;
+    const int nst = a.pre_stile[a.n];
+    if (g == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
+    if (g >= nst) {
+        expire_count(a, g - nst, (int *)lds);
+        return;
+    }
+    const int s = find_stream(a.pre_stile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    const int tile0 = (g - a.pre_stile[s]) * kTile;
+    uint64_t v[8];
+    sort_tile_keys(S, io.in, S.S, tile0, S.mut->u * S.S, v, lds);
+    // coalesced store through LDS
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; e++) lds[threadIdx.x * 8 + e] = v[e];
+    __syncthreads();
+    const int nvalid = min(kTile, S.S - tile0);
+    for (int i = threadIdx.x; i < nvalid; i += kTileThreads) S.bkey0[tile0 + i] = lds[i];
+}
+
+// Number of keys below `key` among lds[0, n), n <= kTile (branch-free
+// binary search over a sorted LDS array).
+__device__ __forceinline__ int lds_lower_bound(const uint64_t *lds, int n, uint64_t key) {
+    int pos = 0;
+#pragma unroll
+    for (int step = kTile; step > 0; step >>= 1)
+        if (pos + step <= n && lds[pos + step - 1] < key) pos += step;
+    return pos;
+}
+
+// Ordered compaction of the survivors into the A arrays. Each tile's offset
+// is the sum of the preceding tiles' counts (no spin-waiting look-back).
+__device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *lds4) {
+    const int s = find_stream(a.pre_wtile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const int tile = g - a.pre_wtile[s];
+    const int p0 = tile * kWTile;
+    const int lo_slot = S.mut->u * S.S;
+    int part = 0;
+    for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_a[t];
+    int run = block_sum(part, lds4);
+    const int dim = S.dim;
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
+        const int p = p0 + r * kTileThreads + threadIdx.x;
+        uint64_t key = 0;
+        bool f = false;
+        if (p < S.W) {
+            key = S.skey[p];
+            f = survives(S, key, lo_slot);
+        }
+        int tot;
+        const int rank = block_rank_flags(f, &tot, lds4);
+        if (f) {
+            const int d = run + rank;
+            S.akey[d] = key;
+            S.ax[d] = S.sx[p];
+            if (dim > 1) S.ay[d] = S.sy[p];
+            if (dim > 2) S.az[d] = S.sz[p];
+        }
+        run += tot;
+    }
+}
+
+// Launch 2 of a slide: blocks [0, stride tiles) place every key of their
+// sorted tile at its rank in the whole stride (its index in the tile plus,
+// for each other tile, the number of that tile's keys below it) and write
+// keys and coordinates to the B1 arrays - the stride sorted by (cell, slot);
+// the remaining blocks compact the window's survivors.
+__global__ __launch_bounds__(kTileThreads) void k_rank_compact(BatchArgs a) {
+    __shared__ uint64_t lds[kTile];
+    const int g = blockIdx.x;
+    const int nst = a.pre_stile[a.n];
+    if (g >= nst) {
+        expire_compact(a, g - nst, (int *)lds);
+        return;
+    }
+    const int s = find_stream(a.pre_stile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    const int t0 = a.pre_stile[s], nt = a.pre_stile[s + 1] - t0;
+    const int tile = g - t0, tile0 = tile * kTile;
+    const int nvalid = min(kTile, S.S - tile0);
+    uint64_t key[8];
+    int rank[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = threadIdx.x + e * kTileThreads;
+        key[e] = i < nvalid ? S.bkey0[tile0 + i] : ~0ull;
+        rank[e] = i;
+    }
+    for (int t = 0; t < nt; t++) {
+        if (t == tile) continue;
+        const int n2 = min(kTile, S.S - t * kTile);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n2; i += kTileThreads) lds[i] = S.bkey0[t * kTile + i];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; e++) rank[e] += lds_lower_bound(lds, n2, key[e]);
+    }
+    const uint32_t m = slot_mask(S);
+    const int slot0 = S.mut->u * S.S, dim = S.dim;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = threadIdx.x + e * kTileThreads;
+        if (i < nvalid) {
+            const int r = rank[e];
+            const int gi = (int)((uint32_t)key[e] & m) - slot0;
+            const float *p = io.in + (size_t)gi * dim;
+            S.bkey1[r] = key[e];
+            S.bx1[r] = p[0];
+            if (dim > 1) S.by1[r] = p[1];
+            if (dim > 2) S.bz1[r] = p[2];
+        }
     }
 }
 
@@ -381,17 +551,20 @@ __device__ __forceinline__ int merge_path_wave(const uint64_t *A, int la, const 
     return lo;
 }
 
+template <int T>
 struct MergeLds {
-    uint64_t in[kTile];
-    uint64_t out[kTile];
-    int32_t src[kTile];  // >= 0: index into A; < 0: ~index into B
+    uint64_t in[T];
+    uint64_t out[T];
+    int32_t src[T];  // >= 0: index into A; < 0: ~index into B
     int32_t sp[2];
     uint64_t pred;
 };
 
-// Merges outputs [d0, d1) of merge(A, B) into m.out / m.src (block-wide).
+// Merges outputs [d0, d1) (d1 - d0 <= T) of merge(A, B) into m.out / m.src
+// (block-wide).
+template <int T>
 __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint64_t *B, int lb, int d0,
-                                           int d1, MergeLds &m, int *a0_out, int *b0_out) {
+                                           int d1, MergeLds<T> &m, int *a0_out, int *b0_out) {
     const int tid = threadIdx.x;
     if (tid < 128) {
         const int sp = merge_path_wave(A, la, B, lb, tid < 64 ? d0 : d1);
@@ -405,7 +578,7 @@ __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint
     for (int i = tid; i < na; i += kTileThreads) m.in[i] = A[a0 + i];
     for (int i = tid; i < nb; i += kTileThreads) m.in[na + i] = B[b0 + i];
     __syncthreads();
-    constexpr int per = kTile / kTileThreads;
+    constexpr int per = T / kTileThreads;
     const int diag = tid * per;
     if (diag < n) {
         int lo = max(0, diag - nb), hi = min(diag, na);
@@ -433,40 +606,20 @@ __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint
     *b0_out = b0;
 }
 
-// One pass of the bottom-up merge sort: merges runs of length L pairwise.
-// INIT: first window, ping-pong between akey (sel 0: a->s) and skey (sel 1:
-// s->a). Otherwise the stride, between bkey0 (sel 0: b0->b1) and bkey1.
-template <bool INIT>
-__global__ __launch_bounds__(kTileThreads) void k_merge_pass(BatchArgs a, int L, int sel) {
-    __shared__ MergeLds m;
+// One pass of the bottom-up merge sort of the first window: merges runs of
+// length L pairwise, ping-pong between akey (sel 0: a->s) and skey (sel 1).
+__global__ __launch_bounds__(kTileThreads) void k_init_merge(BatchArgs a, int L, int sel) {
+    __shared__ MergeLds<kTile> m;
     const int g = blockIdx.x;
-    const int32_t *pre = INIT ? a.pre_wtile : a.pre_stile;
-    const int s = find_stream(pre, a.n, g);
+    const int s = find_stream(a.pre_itile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int len = INIT ? S.W : S.S;
-    const int o0 = (g - pre[s]) * kTile;
+    const int len = S.W;
+    const int o0 = (g - a.pre_itile[s]) * kTile;
     const int o1 = min(o0 + kTile, len);
-    const uint64_t *sk;
-    const float *sx, *sy, *sz;
-    uint64_t *dk;
-    float *dx, *dy, *dz;
-    if (INIT) {
-        if (sel == 0) {
-            sk = S.akey; sx = S.ax; sy = S.ay; sz = S.az;
-            dk = S.skey; dx = S.sx; dy = S.sy; dz = S.sz;
-        } else {
-            sk = S.skey; sx = S.sx; sy = S.sy; sz = S.sz;
-            dk = S.akey; dx = S.ax; dy = S.ay; dz = S.az;
-        }
-    } else {
-        if (sel == 0) {
-            sk = S.bkey0; sx = S.bx0; sy = S.by0; sz = S.bz0;
-            dk = S.bkey1; dx = S.bx1; dy = S.by1; dz = S.bz1;
-        } else {
-            sk = S.bkey1; sx = S.bx1; sy = S.by1; sz = S.bz1;
-            dk = S.bkey0; dx = S.bx0; dy = S.by0; dz = S.bz0;
-        }
-    }
+    const uint64_t *sk = sel == 0 ? S.akey : S.skey;
+    const float *sx = sel == 0 ? S.ax : S.sx, *sy = sel == 0 ? S.ay : S.sy, *sz = sel == 0 ? S.az : S.sz;
+    uint64_t *dk = sel == 0 ? S.skey : S.akey;
+    float *dx = sel == 0 ? S.sx : S.ax, *dy = sel == 0 ? S.sy : S.ay, *dz = sel == 0 ? S.sz : S.az;
     const long long L2 = 2ll * L;
     const int base = (int)((o0 / L2) * L2);
     const int la = min(L, len - base);
@@ -485,80 +638,21 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_pass(BatchArgs a, int L,
 }
 
 // ---------------------------------------------------------------------------
-// window update: drop the expiring stride, merge the new one
-
-__device__ __forceinline__ bool survives(const StreamDev &S, uint64_t key, int lo_slot) {
-    const int slot = (int)((uint32_t)key & slot_mask(S));
-    return slot < lo_slot || slot >= lo_slot + S.S;
-}
-
-// Survivors per window tile (the reference's stable_partition,
-// thrust_helper.cu:37-39, split into count + compact).
-__global__ __launch_bounds__(kTileThreads) void k_expire_count(BatchArgs a) {
-    __shared__ int lds4[4];
-    const int g = blockIdx.x;
-    const int s = find_stream(a.pre_wtile, a.n, g);
-    const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
-    const int p0 = tile * kTile, p1 = min(p0 + kTile, S.W);
-    const int lo_slot = S.mut->u * S.S;
-    int cnt = 0;
-    for (int p = p0 + threadIdx.x; p < p1; p += kTileThreads) cnt += survives(S, S.skey[p], lo_slot);
-    cnt = block_sum(cnt, lds4);
-    if (threadIdx.x == 0) S.tile_a[tile] = cnt;
-}
-
-// Ordered compaction of the survivors into the A arrays. Each tile's offset
-// is the sum of the preceding tiles' counts (no spin-waiting look-back).
-__global__ __launch_bounds__(kTileThreads) void k_expire_compact(BatchArgs a) {
-    __shared__ int lds4[4];
-    const int g = blockIdx.x;
-    const int s = find_stream(a.pre_wtile, a.n, g);
-    const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
-    const int p0 = tile * kTile;
-    const int lo_slot = S.mut->u * S.S;
-    int part = 0;
-    for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_a[t];
-    int run = block_sum(part, lds4);
-    const int dim = S.dim;
-    for (int r = 0; r < kTile / kTileThreads; r++) {
-        const int p = p0 + r * kTileThreads + threadIdx.x;
-        uint64_t key = 0;
-        bool f = false;
-        if (p < S.W) {
-            key = S.skey[p];
-            f = survives(S, key, lo_slot);
-        }
-        int tot;
-        const int rank = block_rank_flags(f, &tot, lds4);
-        if (f) {
-            const int d = run + rank;
-            S.akey[d] = key;
-            S.ax[d] = S.sx[p];
-            if (dim > 1) S.ay[d] = S.sy[p];
-            if (dim > 2) S.az[d] = S.sz[p];
-        }
-        run += tot;
-    }
-}
+// window update: merge the survivors with the sorted stride
 
 // merge(survivors, sorted stride) -> window, plus cell starts per tile
 // (thrust::merge at thrust_helper.cu:47-50 and the run starts of
 // compute_offsets_of_cells, aabb.cu:377-395).
-__global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a, int bsel) {
-    __shared__ MergeLds m;
+__global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
+    __shared__ MergeLds<kWTile> m;
     __shared__ int lds4[4];
     const int g = blockIdx.x;
     const int s = find_stream(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
     const int tile = g - a.pre_wtile[s];
-    const int o0 = tile * kTile, o1 = min(o0 + kTile, S.W);
+    const int o0 = tile * kWTile, o1 = min(o0 + kWTile, S.W);
     const int la = S.W - S.S, lb = S.S;
-    const uint64_t *B = bsel ? S.bkey1 : S.bkey0;
-    const float *bx = bsel ? S.bx1 : S.bx0;
-    const float *by = bsel ? S.by1 : S.by0;
-    const float *bz = bsel ? S.bz1 : S.bz0;
+    const uint64_t *B = S.bkey1;
     int a0, b0;
     merge_tile(S.akey, la, B, lb, o0, o1, m, &a0, &b0);
     if (threadIdx.x == 0) {
@@ -580,9 +674,9 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a, int b
             if (dim > 2) S.sz[o0 + i] = S.az[sidx];
         } else {
             const int j = ~sidx;
-            S.sx[o0 + i] = bx[j];
-            if (dim > 1) S.sy[o0 + i] = by[j];
-            if (dim > 2) S.sz[o0 + i] = bz[j];
+            S.sx[o0 + i] = S.bx1[j];
+            if (dim > 1) S.sy[o0 + i] = S.by1[j];
+            if (dim > 2) S.sz[o0 + i] = S.bz1[j];
         }
         const uint64_t prev = i > 0 ? m.out[i - 1] : m.pred;
         starts += (o0 + i == 0) || ((key >> sb) != (prev >> sb));
@@ -597,21 +691,21 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a, int b
 // so no sort and no host round trip (dbscan.cpp:549) are needed.
 __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     __shared__ int lds4[4];
-    __shared__ int newc[kTile];        // this tile's new cells ...
-    __shared__ uint64_t newk[kTile];   // ... and their keys
+    __shared__ int newc[kWTile];        // this tile's new cells ...
+    __shared__ uint64_t newk[kWTile];   // ... and their keys
     const int g = blockIdx.x;
     const int s = find_stream(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
     const int tile = g - a.pre_wtile[s];
     const int ntile = a.pre_wtile[s + 1] - a.pre_wtile[s];
-    const int p0 = tile * kTile;
+    const int p0 = tile * kWTile;
     const uint32_t epoch = S.mut->epoch;
     const int sb = S.slot_bits;
     int part = 0;
     for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_b[t];
     const int base = block_sum(part, lds4);
     int run = 0;
-    for (int r = 0; r < kTile / kTileThreads; r++) {
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
         const bool valid = p < S.W;
         uint64_t ck = 0;
@@ -846,10 +940,15 @@ __device__ __forceinline__ void walk_candidates(const StreamDev &S, int c, Visit
     }
 }
 
-// Query ownership of a block of a point-parallel kernel with LPQ lanes per
-// query: block g covers chunk g / LPQ; each query is served by LPQ adjacent
-// lanes that split its stencil rows. More lanes per query shorten each
-// thread's chain of dependent loads when a launch has few queries.
+// Query ownership of a virtual block of a point-parallel kernel with LPQ
+// lanes per query: virtual block g covers chunk g / LPQ; each query is served
+// by LPQ adjacent lanes that split its stencil rows. More lanes per query
+// shorten each thread's chain of dependent loads when a launch has few
+// queries.
+//
+// The point-parallel kernels are persistent: a bounded grid of one-wave
+// blocks walks the virtual blocks, since thousands of short one-wave
+// workgroups are dispatched more slowly than they run.
 template <int LPQ>
 struct QueryLane {
     ChunkCtx x;
@@ -858,9 +957,9 @@ struct QueryLane {
 };
 
 template <int LPQ>
-__device__ __forceinline__ QueryLane<LPQ> query_lane(const BatchArgs &a) {
+__device__ __forceinline__ QueryLane<LPQ> query_lane(const BatchArgs &a, int vb) {
     QueryLane<LPQ> q;
-    const int chunk = blockIdx.x / LPQ, part = blockIdx.x % LPQ;
+    const int chunk = vb / LPQ, part = vb % LPQ;
     q.x = chunk_of(a, chunk);
     q.sub = lane_id() % LPQ;
     q.p = q.x.p0 + part * (64 / LPQ) + lane_id() / LPQ;
@@ -889,8 +988,8 @@ __device__ __forceinline__ int group_min(int v) {
 // once its own count reaches min_pts. Queries of one cell are adjacent and
 // read the same candidates.
 template <int LPQ>
-__global__ __launch_bounds__(64) void k_count(BatchArgs a) {
-    const QueryLane<LPQ> ql = query_lane<LPQ>(a);
+__device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
     const StreamDev &S = a.s[ql.x.s];
     const int p = ql.p;
     const bool valid = p < ql.x.p1;
@@ -916,10 +1015,16 @@ __global__ __launch_bounds__(64) void k_count(BatchArgs a) {
             ql.sub, LPQ);
     }
     cnt = group_sum<LPQ>(cnt);
-    if (!valid || ql.sub != 0) return;
-    const int sl = slot_of(S, p);
-    S.parent[sl] = sl;
-    S.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
+    if (valid && ql.sub == 0) {
+        const int sl = slot_of(S, p);
+        S.parent[sl] = sl;
+        S.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
+    }
+}
+
+template <int LPQ>
+__global__ __launch_bounds__(64) void k_count(BatchArgs a, int nvb) {
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) count_vblock<LPQ>(a, vb);
 }
 
 // First (minimum-slot) core of every cell - its union-find representative -
@@ -1354,8 +1459,8 @@ __global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
 // grids whose cells are not dense-safe): each unites with every core within
 // eps (the reference's core branch of __intersection__hybrid_radius_sphere,
 // dbscan.cu:364-369), through the representatives of safe cells.
-__global__ __launch_bounds__(64) void k_union_points(BatchArgs a) {
-    const ChunkCtx x = chunk_of(a, blockIdx.x);
+__device__ __forceinline__ void union_points_chunk(const BatchArgs &a, int g) {
+    const ChunkCtx x = chunk_of(a, g);
     const StreamDev &S = a.s[x.s];
     const int p = x.p0 + lane_id();
     if (p >= x.p1) return;
@@ -1375,15 +1480,19 @@ __global__ __launch_bounds__(64) void k_union_points(BatchArgs a) {
     });
 }
 
+__global__ __launch_bounds__(64) void k_union_points(BatchArgs a, int nvb) {
+    for (int g = blockIdx.x; g < nvb; g += gridDim.x) union_points_chunk(a, g);
+}
+
 // Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
 // root of the core's representative. Block 0 of each stream also advances the
 // ring cursor and hash epoch; no later kernel of the slide reads them.
-__global__ __launch_bounds__(64) void k_flatten(BatchArgs a) {
-    const ChunkCtx x = chunk_of(a, blockIdx.x);
+__device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
+    const ChunkCtx x = chunk_of(a, g);
     const StreamDev &S = a.s[x.s];
     const IO io = a.use_io1 ? a.io1 : a.io[x.s];
     const int p = x.p0 + lane_id();
-    if (blockIdx.x == a.pre_chunk[x.s] && lane_id() == 0) {
+    if (g == a.pre_chunk[x.s] && lane_id() == 0) {
         StreamMut *m = S.mut;
         m->t = m->t + 1;
         m->u = m->u + 1 == S.units ? 0 : m->u + 1;
@@ -1398,12 +1507,16 @@ __global__ __launch_bounds__(64) void k_flatten(BatchArgs a) {
     io.cid[slot] = v;
 }
 
+__global__ __launch_bounds__(64) void k_flatten(BatchArgs a, int nvb) {
+    for (int g = blockIdx.x; g < nvb; g += gridDim.x) flatten_chunk(a, g);
+}
+
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
 // deterministic rule: the minimum canonical id among cores within eps),
 // point-parallel over the non-core points, LPQ lanes per point.
 template <int LPQ>
-__global__ __launch_bounds__(64) void k_border(BatchArgs a) {
-    const QueryLane<LPQ> ql = query_lane<LPQ>(a);
+__device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
     const StreamDev &S = a.s[ql.x.s];
     const int p = ql.p;
     bool mine = p < ql.x.p1;
@@ -1426,11 +1539,17 @@ __global__ __launch_bounds__(64) void k_border(BatchArgs a) {
             ql.sub, LPQ);
     }
     best = group_min<LPQ>(best);
-    if (!mine || ql.sub != 0) return;
-    const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
-    const int slot = slot_of(S, p);
-    io.label[slot] = best != INT_MAX ? 1 : 2;
-    io.cid[slot] = best != INT_MAX ? best : slot;
+    if (mine && ql.sub == 0) {
+        const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
+        const int slot = slot_of(S, p);
+        io.label[slot] = best != INT_MAX ? 1 : 2;
+        io.cid[slot] = best != INT_MAX ? best : slot;
+    }
+}
+
+template <int LPQ>
+__global__ __launch_bounds__(64) void k_border(BatchArgs a, int nvb) {
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) border_vblock<LPQ>(a, vb);
 }
 
 __global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
@@ -1474,9 +1593,11 @@ static inline hipError_t mark(hipEvent_t *ev, uint32_t mask, int idx, hipStream_
 }
 
 template <int LPQ>
-static hipError_t launch_count(const BatchArgs &a, int chunks, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    if (e0) hipExtLaunchKernelGGL(k_count<LPQ>, dim3(chunks * LPQ), dim3(64), 0, st, e0, e1, 0, a);
-    else hipLaunchKernelGGL(k_count<LPQ>, dim3(chunks * LPQ), dim3(64), 0, st, a);
+static hipError_t launch_count(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t e0,
+                               hipEvent_t e1) {
+    const int nvb = sh.chunks * LPQ, grid = std::min(nvb, sh.wave_blocks);
+    if (e0) hipExtLaunchKernelGGL(k_count<LPQ>, dim3(grid), dim3(64), 0, st, e0, e1, 0, a, nvb);
+    else hipLaunchKernelGGL(k_count<LPQ>, dim3(grid), dim3(64), 0, st, a, nvb);
     return hipGetLastError();
 }
 
@@ -1486,17 +1607,13 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
 #define PH_BEGIN(ph) if ((e = mark(ev, phase_mask, 2 * (ph), st)) != hipSuccess) return e
 #define PH_END(ph) if ((e = mark(ev, phase_mask, 2 * (ph) + 1, st)) != hipSuccess) return e
     PH_BEGIN(kPhStrideSort);
-    RTDBS_LAUNCH(k_stride_tiles<false>, sh.stiles, kTileThreads, a);
+    RTDBS_LAUNCH(k_sort_expire, sh.stiles + sh.wtiles, kTileThreads, a);
     PH_END(kPhStrideSort);
     PH_BEGIN(kPhMerge);
-    for (int p = 0; p < sh.passes; p++) RTDBS_LAUNCH(k_merge_pass<false>, sh.stiles, kTileThreads, a, kTile << p, p & 1);
+    RTDBS_LAUNCH(k_rank_compact, sh.stiles + sh.wtiles, kTileThreads, a);
     PH_END(kPhMerge);
-    PH_BEGIN(kPhExpire);
-    RTDBS_LAUNCH(k_expire_count, sh.wtiles, kTileThreads, a);
-    RTDBS_LAUNCH(k_expire_compact, sh.wtiles, kTileThreads, a);
-    PH_END(kPhExpire);
     PH_BEGIN(kPhMergeFinal);
-    RTDBS_LAUNCH(k_merge_final, sh.wtiles, kTileThreads, a, sh.passes & 1);
+    RTDBS_LAUNCH(k_merge_final, sh.wtiles, kTileThreads, a);
     PH_END(kPhMergeFinal);
     PH_BEGIN(kPhCells);
     RTDBS_LAUNCH(k_cells, sh.wtiles, kTileThreads, a);
@@ -1510,23 +1627,23 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         // their own between it and its neighbours.
         const bool timed = ev && ((phase_mask >> kPhCount) & 1u);
         const hipEvent_t e0 = timed ? ev[2 * kPhCount] : nullptr, e1 = timed ? ev[2 * kPhCount + 1] : nullptr;
-        if (sh.lpq >= 4) e = launch_count<4>(a, sh.chunks, st, e0, e1);
-        else if (sh.lpq == 2) e = launch_count<2>(a, sh.chunks, st, e0, e1);
-        else e = launch_count<1>(a, sh.chunks, st, e0, e1);
+        if (sh.lpq >= 4) e = launch_count<4>(a, sh, st, e0, e1);
+        else if (sh.lpq == 2) e = launch_count<2>(a, sh, st, e0, e1);
+        else e = launch_count<1>(a, sh, st, e0, e1);
         if (e != hipSuccess) return e;
     }
     PH_BEGIN(kPhUnion);
     RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
     RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
-    RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
+    RTDBS_LAUNCH(k_union_points, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
-    RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
+    RTDBS_LAUNCH(k_flatten, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
-    if (sh.lpq >= 4) RTDBS_LAUNCH(k_border<4>, sh.chunks * 4, 64, a);
-    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border<2>, sh.chunks * 2, 64, a);
-    else RTDBS_LAUNCH(k_border<1>, sh.chunks, 64, a);
+    if (sh.lpq >= 4) RTDBS_LAUNCH(k_border<4>, std::min(sh.chunks * 4, sh.wave_blocks), 64, a, sh.chunks * 4);
+    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border<2>, std::min(sh.chunks * 2, sh.wave_blocks), 64, a, sh.chunks * 2);
+    else RTDBS_LAUNCH(k_border<1>, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END
@@ -1534,8 +1651,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
 }
 
 hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipStream_t st) {
-    RTDBS_LAUNCH(k_stride_tiles<true>, itiles, kTileThreads, a);
-    for (int p = 0; p < passes; p++) RTDBS_LAUNCH(k_merge_pass<true>, itiles, kTileThreads, a, kTile << p, p & 1);
+    RTDBS_LAUNCH(k_init_tiles, itiles, kTileThreads, a);
+    for (int p = 0; p < passes; p++) RTDBS_LAUNCH(k_init_merge, itiles, kTileThreads, a, kTile << p, p & 1);
     return hipSuccess;
 }
 
