@@ -109,7 +109,8 @@ struct LaunchShape {
     int32_t persist_blocks;  // 256-thread blocks for the persistent row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
     int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
-    int32_t wave_blocks;     // grid cap of the persistent one-wave point kernels
+    int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels
+    int32_t cellpar;         // 1: cell-parallel count/border, 0: point-parallel
 };
 
 enum Phase {

@@ -540,16 +540,22 @@ void batch_free(rtdbs_batch *b) {
     delete b;
 }
 
+// Integer launch-shape override from the environment, `def` when unset or
+// outside [lo, hi].
+static int env_int(const char *name, int def, int lo, int hi) {
+    const char *e = std::getenv(name);
+    if (!e || !*e) return def;
+    const int x = std::atoi(e);
+    return (x >= lo && x <= hi) ? x : def;
+}
+
+// Windows up to this many points in one launch use the cell-parallel count
+// and border kernels.
+constexpr long long kCellParMaxPoints = 1000000;
+
 // One-wave workgroups per CU for the persistent cell-level merge
 // (RTDBS_WAVES_PER_CU overrides, 1..64).
-static int waves_per_cu() {
-    static int v = [] {
-        const char *e = std::getenv("RTDBS_WAVES_PER_CU");
-        int x = e ? std::atoi(e) : 0;
-        return (x >= 1 && x <= 64) ? x : 16;
-    }();
-    return v;
-}
+static int waves_per_cu() { return env_int("RTDBS_WAVES_PER_CU", 16, 1, 64); }
 
 // Uploads descriptors and prefix tables; drops a captured graph. Called when
 // the batch is built and whenever a member stream was re-initialised.
@@ -582,10 +588,14 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
-    // One-wave blocks: the point kernels take one block per virtual block (a
-    // bounded persistent grid measured slower, the waves' load chains being
-    // the limit, not their dispatch); the cell-level merge is capped.
-    b->shape.wave_blocks = 1 << 30;
+    // Count and border: cell-parallel (coalesced candidate loads, more
+    // instructions per query) while a launch is small and latency-bound,
+    // point-parallel once its total window saturates the device.
+    // RTDBS_NEIGHBOR_MODE: 1 forces cell-parallel, 2 point-parallel.
+    const int nmode = env_int("RTDBS_NEIGHBOR_MODE", 0, 0, 2);
+    b->shape.cellpar = nmode == 1 || (nmode == 0 && cells_total <= kCellParMaxPoints);
+    b->shape.cellpar_blocks = (int)std::max(
+        1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", 16, 1, 1024), (cells_total + 3) / 4));
     b->shape.cell_blocks = (int)std::max(
         1ll, std::min((long long)device_cus() * waves_per_cu(), (cells_total + kUGCells - 1) / kUGCells));
     // lanes per query: enough threads to cover the device several times over,

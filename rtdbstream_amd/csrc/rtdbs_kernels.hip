@@ -940,15 +940,10 @@ __device__ __forceinline__ void walk_candidates(const StreamDev &S, int c, Visit
     }
 }
 
-// Query ownership of a virtual block of a point-parallel kernel with LPQ
-// lanes per query: virtual block g covers chunk g / LPQ; each query is served
-// by LPQ adjacent lanes that split its stencil rows. More lanes per query
-// shorten each thread's chain of dependent loads when a launch has few
-// queries.
-//
-// The point-parallel kernels are persistent: a bounded grid of one-wave
-// blocks walks the virtual blocks, since thousands of short one-wave
-// workgroups are dispatched more slowly than they run.
+// Query ownership of a block of a point-parallel kernel with LPQ lanes per
+// query: block g covers chunk g / LPQ; each query is served by LPQ adjacent
+// lanes that split its stencil rows. More lanes per query shorten each
+// thread's chain of dependent loads when a launch has few queries.
 template <int LPQ>
 struct QueryLane {
     ChunkCtx x;
@@ -980,9 +975,172 @@ __device__ __forceinline__ int group_min(int v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// cell-parallel neighbourhood kernels: one wave per cell. The lanes hold 64
+// of the cell's stencil candidates at a time - consecutive positions of its
+// rows, so the loads coalesce and each candidate is read once per cell rather
+// than once per query - and every query of the cell is tested against all of
+// them with one ballot.
+
+constexpr int kCellWaves = 4;  // waves (cells) per block
+
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The cell's stencil rows, one per lane (lane < R): start position `rx` and
+// exclusive prefix `pre` of the row lengths; returns the candidate total.
+__device__ __forceinline__ int cell_rows(const StreamDev &S, int A, int *rx, int *pre) {
+    const int lane = lane_id();
+    int x = 0, cnt = 0;
+    if (lane < S.R) {
+        const int2 r = S.rows[(size_t)A * S.R + lane];
+        x = r.x;
+        cnt = r.y - r.x;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    *rx = x;
+    *pre = incl - cnt;
+    return __shfl(incl, 63);
+}
+
+// Position of flat candidate j (< total) of the cell's rows.
+__device__ __forceinline__ int cell_candidate(int rx, int pre, int R, int j) {
+    const int r = row_of(pre, R, j);
+    return __shfl(rx, r) + (j - __shfl(pre, r));
+}
+
 // Range count = core identification (the reference's __raygen__grid +
 // __intersection__grid, dbscan.cu:155-194,250-270, and set_label,
-// aabb.cu:575-587). Point-parallel: dense cells are core without counting
+// aabb.cu:575-587). Dense cells are core without counting (all pairs within
+// eps). Otherwise each query counts its neighbours, itself included, and
+// stops once the count reaches min_pts.
+__device__ void count_cell(const StreamDev &S, int A) {
+    const int lane = lane_id();
+    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
+    if (S.cflag[A] & kCellDense) {
+        for (int i = lane; i < nA; i += 64) {
+            const int p = cs + i, sl = slot_of(S, p);
+            S.parent[sl] = sl;
+            S.flags[p] = kFlagCore | kFlagDense;
+        }
+        return;
+    }
+    const int R = S.R, dim = S.dim, minpts = S.min_pts;
+    const float e2 = S.eps2;
+    int rx, pre;
+    const int TC = cell_rows(S, A, &rx, &pre);
+    for (int q0 = 0; q0 < nA; q0 += 64) {
+        const int nq = min(64, nA - q0);
+        const int pq = cs + q0 + lane;
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        if (lane < nq) {
+            qx = S.sx[pq];
+            if (dim > 1) qy = S.sy[pq];
+            if (dim > 2) qz = S.sz[pq];
+        }
+        int mine = 0;
+        uint64_t open = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
+        for (int jb = 0; jb < TC && open; jb += 64) {
+            const int j = jb + lane;
+            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
+            const float cx = S.sx[pos];
+            const float cy = dim > 1 ? S.sy[pos] : 0.f;
+            const float cz = dim > 2 ? S.sz[pos] : 0.f;
+            const bool v = j < TC;
+            uint64_t m = open;
+            while (m) {
+                const int qi = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const int c =
+                    __popcll(__ballot(v && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2));
+                if (lane == qi) mine += c;
+            }
+            open &= ~__ballot(mine >= minpts);
+        }
+        if (lane < nq) {
+            const int sl = slot_of(S, pq);
+            S.parent[sl] = sl;
+            S.flags[pq] = mine >= minpts ? kFlagCore : 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
+    const int total = a.cell_pre[a.n];
+    const int nw = gridDim.x * kCellWaves;
+    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+        const CellCtx x = cell_of(a, gc);
+        count_cell(a.s[x.s], x.c);
+    }
+}
+
+// Border / noise (the reference's border claim, dbscan.cu:370-377, with the
+// deterministic rule: the minimum canonical id among cores within eps) for
+// the non-core points of a cell; cells without them are skipped.
+__device__ void border_cell(const StreamDev &S, const IO &io, int A) {
+    const int lane = lane_id();
+    if (S.cflag[A] & kCellDense) return;
+    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
+    const int R = S.R, dim = S.dim;
+    const float e2 = S.eps2;
+    int rx = 0, pre = 0, TC = -1;
+    for (int q0 = 0; q0 < nA; q0 += 64) {
+        const int nq = min(64, nA - q0);
+        const int pq = cs + q0 + lane;
+        const bool isq = lane < nq && !(S.flags[pq] & kFlagCore);
+        const uint64_t open = __ballot(isq);
+        if (open == 0ull) continue;
+        if (TC < 0) TC = cell_rows(S, A, &rx, &pre);
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        if (isq) {
+            qx = S.sx[pq];
+            if (dim > 1) qy = S.sy[pq];
+            if (dim > 2) qz = S.sz[pq];
+        }
+        int best = INT_MAX;
+        for (int jb = 0; jb < TC; jb += 64) {
+            const int j = jb + lane;
+            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
+            const float cx = S.sx[pos];
+            const float cy = dim > 1 ? S.sy[pos] : 0.f;
+            const float cz = dim > 2 ? S.sz[pos] : 0.f;
+            const bool core = j < TC && (S.flags[pos] & kFlagCore);
+            const int cid = core ? S.canon[pos] : INT_MAX;
+            uint64_t m = open;
+            while (m) {
+                const int qi = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const bool hit = core && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
+                if (__ballot(hit) == 0ull) continue;
+                const int mn = wave_min(hit ? cid : INT_MAX);
+                if (lane == qi) best = min(best, mn);
+            }
+        }
+        if (isq) {
+            const int slot = slot_of(S, pq);
+            io.label[slot] = best != INT_MAX ? 1 : 2;
+            io.cid[slot] = best != INT_MAX ? best : slot;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
+    const int total = a.cell_pre[a.n];
+    const int nw = gridDim.x * kCellWaves;
+    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+        const CellCtx x = cell_of(a, gc);
+        border_cell(a.s[x.s], a.use_io1 ? a.io1 : a.io[x.s], x.c);
+    }
+}
+
+// Range count, point-parallel form (large batches, where the device is
+// saturated and per-query work matters more than coalescing): dense cells are core without counting
 // (all pairs within eps); otherwise the LPQ lanes of a query count its
 // neighbours (itself included) over its cell's stencil rows, each stopping
 // once its own count reaches min_pts. Queries of one cell are adjacent and
@@ -1023,8 +1181,48 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
 }
 
 template <int LPQ>
-__global__ __launch_bounds__(64) void k_count(BatchArgs a, int nvb) {
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) count_vblock<LPQ>(a, vb);
+__global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
+    count_vblock<LPQ>(a, blockIdx.x);
+}
+
+// Border / noise, point-parallel form (large batches): LPQ lanes per
+// non-core point.
+template <int LPQ>
+__device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
+    const StreamDev &S = a.s[ql.x.s];
+    const int p = ql.p;
+    bool mine = p < ql.x.p1;
+    if (mine) mine = !(S.flags[p] & kFlagCore);
+    const int dim = S.dim;
+    const float e2 = S.eps2;
+    int best = INT_MAX;
+    if (mine) {
+        const int c = S.pcell[p];
+        const float qx = S.sx[p];
+        const float qy = dim > 1 ? S.sy[p] : 0.f;
+        const float qz = dim > 2 ? S.sz[p] : 0.f;
+        walk_candidates(
+            S, c,
+            [&](int j, float cx, float cy, float cz) {
+                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
+                    best = min(best, S.canon[j]);
+                return false;
+            },
+            ql.sub, LPQ);
+    }
+    best = group_min<LPQ>(best);
+    if (mine && ql.sub == 0) {
+        const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
+        const int slot = slot_of(S, p);
+        io.label[slot] = best != INT_MAX ? 1 : 2;
+        io.cid[slot] = best != INT_MAX ? best : slot;
+    }
+}
+
+template <int LPQ>
+__global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
+    border_vblock<LPQ>(a, blockIdx.x);
 }
 
 // First (minimum-slot) core of every cell - its union-find representative -
@@ -1480,9 +1678,7 @@ __device__ __forceinline__ void union_points_chunk(const BatchArgs &a, int g) {
     });
 }
 
-__global__ __launch_bounds__(64) void k_union_points(BatchArgs a, int nvb) {
-    for (int g = blockIdx.x; g < nvb; g += gridDim.x) union_points_chunk(a, g);
-}
+__global__ __launch_bounds__(64) void k_union_points(BatchArgs a) { union_points_chunk(a, blockIdx.x); }
 
 // Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
 // root of the core's representative. Block 0 of each stream also advances the
@@ -1507,50 +1703,7 @@ __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
     io.cid[slot] = v;
 }
 
-__global__ __launch_bounds__(64) void k_flatten(BatchArgs a, int nvb) {
-    for (int g = blockIdx.x; g < nvb; g += gridDim.x) flatten_chunk(a, g);
-}
-
-// Border / noise (the reference's border claim, dbscan.cu:370-377, with the
-// deterministic rule: the minimum canonical id among cores within eps),
-// point-parallel over the non-core points, LPQ lanes per point.
-template <int LPQ>
-__device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
-    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
-    const StreamDev &S = a.s[ql.x.s];
-    const int p = ql.p;
-    bool mine = p < ql.x.p1;
-    if (mine) mine = !(S.flags[p] & kFlagCore);
-    const int dim = S.dim;
-    const float e2 = S.eps2;
-    int best = INT_MAX;
-    if (mine) {
-        const int c = S.pcell[p];
-        const float qx = S.sx[p];
-        const float qy = dim > 1 ? S.sy[p] : 0.f;
-        const float qz = dim > 2 ? S.sz[p] : 0.f;
-        walk_candidates(
-            S, c,
-            [&](int j, float cx, float cy, float cz) {
-                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
-                    best = min(best, S.canon[j]);
-                return false;
-            },
-            ql.sub, LPQ);
-    }
-    best = group_min<LPQ>(best);
-    if (mine && ql.sub == 0) {
-        const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
-        const int slot = slot_of(S, p);
-        io.label[slot] = best != INT_MAX ? 1 : 2;
-        io.cid[slot] = best != INT_MAX ? best : slot;
-    }
-}
-
-template <int LPQ>
-__global__ __launch_bounds__(64) void k_border(BatchArgs a, int nvb) {
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) border_vblock<LPQ>(a, vb);
-}
+__global__ __launch_bounds__(64) void k_flatten(BatchArgs a) { flatten_chunk(a, blockIdx.x); }
 
 __global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
     const StreamDev &S = a.s[blockIdx.y];
@@ -1592,12 +1745,12 @@ static inline hipError_t mark(hipEvent_t *ev, uint32_t mask, int idx, hipStream_
     return hipEventRecord(ev[idx], st);
 }
 
-template <int LPQ>
-static hipError_t launch_count(const BatchArgs &a, const LaunchShape &sh, hipStream_t st, hipEvent_t e0,
-                               hipEvent_t e1) {
-    const int nvb = sh.chunks * LPQ, grid = std::min(nvb, sh.wave_blocks);
-    if (e0) hipExtLaunchKernelGGL(k_count<LPQ>, dim3(grid), dim3(64), 0, st, e0, e1, 0, a, nvb);
-    else hipLaunchKernelGGL(k_count<LPQ>, dim3(grid), dim3(64), 0, st, a, nvb);
+// Launches `kernel`; with events, they take the dispatch's own timestamps.
+template <class K, class... Args>
+static hipError_t launch_timed(K kernel, int grid, int block, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                               Args... args) {
+    if (e0) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, args...);
     return hipGetLastError();
 }
 
@@ -1627,23 +1780,25 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         // their own between it and its neighbours.
         const bool timed = ev && ((phase_mask >> kPhCount) & 1u);
         const hipEvent_t e0 = timed ? ev[2 * kPhCount] : nullptr, e1 = timed ? ev[2 * kPhCount + 1] : nullptr;
-        if (sh.lpq >= 4) e = launch_count<4>(a, sh, st, e0, e1);
-        else if (sh.lpq == 2) e = launch_count<2>(a, sh, st, e0, e1);
-        else e = launch_count<1>(a, sh, st, e0, e1);
+        if (sh.cellpar) e = launch_timed(k_count, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
+        else if (sh.lpq >= 4) e = launch_timed(k_count_points<4>, sh.chunks * 4, 64, st, e0, e1, a);
+        else if (sh.lpq == 2) e = launch_timed(k_count_points<2>, sh.chunks * 2, 64, st, e0, e1, a);
+        else e = launch_timed(k_count_points<1>, sh.chunks, 64, st, e0, e1, a);
         if (e != hipSuccess) return e;
     }
     PH_BEGIN(kPhUnion);
     RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
     RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
-    RTDBS_LAUNCH(k_union_points, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
+    RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
-    RTDBS_LAUNCH(k_flatten, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
+    RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
-    if (sh.lpq >= 4) RTDBS_LAUNCH(k_border<4>, std::min(sh.chunks * 4, sh.wave_blocks), 64, a, sh.chunks * 4);
-    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border<2>, std::min(sh.chunks * 2, sh.wave_blocks), 64, a, sh.chunks * 2);
-    else RTDBS_LAUNCH(k_border<1>, std::min(sh.chunks, sh.wave_blocks), 64, a, sh.chunks);
+    if (sh.cellpar) RTDBS_LAUNCH(k_border, sh.cellpar_blocks, 64 * kCellWaves, a);
+    else if (sh.lpq >= 4) RTDBS_LAUNCH(k_border_points<4>, sh.chunks * 4, 64, a);
+    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border_points<2>, sh.chunks * 2, 64, a);
+    else RTDBS_LAUNCH(k_border_points<1>, sh.chunks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END

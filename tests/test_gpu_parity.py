@@ -178,7 +178,9 @@ def test_autotune_choice_is_exact(engine, oracle, tmp_path, monkeypatch):
 
 
 @pytest.mark.parametrize("lpq", [1, 2, 4])
-def test_lanes_per_query_settings_agree(engine, oracle, lpq):
+def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
+    # lanes per query shape the point-parallel count and border kernels
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", "2")
     data = _mixed(3000 + 4 * 300, 2, 42)
     ref_l, ref_c = oracle.stream(data, 3000, 300, 0.2, 5, 4, method="grid")
     s = engine.Stream(2, 3000, 300, 0.2, 5, lanes_per_query=lpq)
@@ -187,3 +189,16 @@ def test_lanes_per_query_settings_agree(engine, oracle, lpq):
         l, c = s.slide(data[3000 + t * 300: 3000 + (t + 1) * 300])
         assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
     s.close()
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("dim,W,S,eps,mp,k,seed", [SMALL[2], SMALL[6], SMALL[9], SMALL[13]])
+def test_neighbour_kernel_forms_agree(engine, oracle, monkeypatch, mode, dim, W, S, eps, mp, k, seed):
+    """Cell-parallel (1) and point-parallel (2) count/border kernels both
+    reproduce the oracle (launches pick one by size; both are forced here)."""
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    slides = 4
+    data = _mixed(W + slides * S, dim, seed)
+    ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
+    got = engine_slides(engine, data, dim, W, S, eps, mp, slides, cell_div=k)
+    assert_parity(got, ref_l, ref_c, "mode %s dim%d W%d k%d" % (mode, dim, W, k))
