@@ -82,6 +82,7 @@ struct StreamDev {
     uint8_t *flags;   // W: kFlagCore | kFlagDense
     int32_t *parent;  // W: union-find over positions, root = minimum slot
     int32_t *canon;   // W: canonical id (min core slot) of core positions
+    int32_t *rep;     // W: union-find element of each position (cell-parallel path)
     int32_t *tile_a;  // ceil(W/kWTile): survivors per tile
     int32_t *tile_b;  // ceil(W/kWTile): cell starts per tile
 };

@@ -115,7 +115,7 @@ struct Layout {
     size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
     size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
     size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cflag, off_fcore, off_flags, off_parent,
-        off_canon, off_tile_a, off_tile_b, off_in, off_label, off_cid;
+        off_canon, off_rep, off_tile_a, off_tile_b, off_in, off_label, off_cid;
     size_t total;
 };
 
@@ -157,6 +157,7 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
     L.off_flags = take(W);
     L.off_parent = take(W * 4);
     L.off_canon = take(W * 4);
+    L.off_rep = take(W * 4);
     L.off_tile_a = take(ntile * 4);
     L.off_tile_b = take(ntile * 4);
     L.off_in = take(W * d * 4);
@@ -272,6 +273,7 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     D.flags = (uint8_t *)(A + L.off_flags);
     D.parent = (int32_t *)(A + L.off_parent);
     D.canon = (int32_t *)(A + L.off_canon);
+    D.rep = (int32_t *)(A + L.off_rep);
     D.tile_a = (int32_t *)(A + L.off_tile_a);
     D.tile_b = (int32_t *)(A + L.off_tile_b);
     s->d_in = (float *)(A + L.off_in);

@@ -1023,18 +1023,23 @@ __device__ __forceinline__ int cell_candidate(int rx, int pre, int R, int j) {
 __device__ void count_cell(const StreamDev &S, int A) {
     const int lane = lane_id();
     const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
-    if (S.cflag[A] & kCellDense) {
+    const uint8_t cf = S.cflag[A];
+    if (cf & kCellDense) {
         for (int i = lane; i < nA; i += 64) {
             const int p = cs + i, sl = slot_of(S, p);
             S.parent[sl] = sl;
             S.flags[p] = kFlagCore | kFlagDense;
         }
+        const int rs = slot_of(S, cs);
+        for (int i = lane; i < nA; i += 64) S.rep[cs + i] = rs;
+        if (lane == 0) S.fcore[A] = cs;
         return;
     }
     const int R = S.R, dim = S.dim, minpts = S.min_pts;
     const float e2 = S.eps2;
     int rx, pre;
     const int TC = cell_rows(S, A, &rx, &pre);
+    int fc = kNoCore;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + lane;
@@ -1068,7 +1073,15 @@ __device__ void count_cell(const StreamDev &S, int A) {
             S.parent[sl] = sl;
             S.flags[pq] = mine >= minpts ? kFlagCore : 0;
         }
+        const uint64_t cm = __ballot(lane < nq && mine >= minpts);
+        if (fc == kNoCore && cm != 0ull) fc = cs + q0 + __ffsll((unsigned long long)cm) - 1;
     }
+    if (lane == 0) S.fcore[A] = fc;  // first (minimum-slot) core: the cell's representative
+    // union-find element of each point: the representative's slot in a safe
+    // cell, else its own slot
+    const bool safe = cf & kCellSafe;
+    const int rs = fc != kNoCore ? slot_of(S, fc) : 0;
+    for (int i = lane; i < nA; i += 64) S.rep[cs + i] = safe ? rs : slot_of(S, cs + i);
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
@@ -1136,6 +1149,112 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
     for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
         border_cell(a.s[x.s], a.use_io1 ? a.io1 : a.io[x.s], x.c);
+    }
+}
+
+// Unites `target` with `rep` for every lane that wants it, once per run of
+// lanes holding the same representative (a safe cell's points are adjacent
+// in a candidate chunk and share one representative).
+__device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, int rep, int target) {
+    const int lane = lane_id();
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return;
+    const uint64_t below = m & ((1ull << lane) - 1ull);
+    const int prev = below ? 63 - __clzll((long long)below) : lane;
+    const int prep = __shfl(rep, prev);
+    if (want && (below == 0ull || prep != rep)) uf_unite(parent, target, rep);
+}
+
+// Cluster merge (the reference's __raygen__cluster + unite,
+// dbscan.cu:279-321,354-394,9-43), cell-parallel: one wave per cell A with
+// cores. Its candidates are the stencil positions before A's own first
+// position - exactly the points of the stencil cells ordered before A, so each
+// unordered pair of cells is tested once, from its later cell - loaded 64 at
+// a time, coalesced. A safe cell A (all its points within eps of each other)
+// is one union-find element, its representative: a candidate core within eps
+// of any core of A joins it. The cores of an unsafe cell are separate
+// elements, and the cell's own earlier points are candidates too. A
+// candidate's element is its safe cell's representative or its own slot.
+__device__ void union_cell(const StreamDev &S, int A) {
+    const int lane = lane_id();
+    const int fcA = S.fcore[A];
+    if (fcA == kNoCore) return;
+    const uint8_t cfA = S.cflag[A];
+    const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
+    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
+    const int R = S.R, dim = S.dim;
+    const float e2 = S.eps2;
+    const int limit = safeA ? cs : cs + nA;
+    int rx = 0, pre = 0, cnt = 0;
+    if (lane < R) {
+        const int2 r = S.rows[(size_t)A * R + lane];
+        rx = r.x;
+        cnt = max(0, min(r.y, limit) - r.x);
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    pre = incl - cnt;
+    const int TC = __shfl(incl, 63);
+    if (TC == 0) return;
+    const int repA = slot_of(S, fcA);
+    for (int q0 = 0; q0 < nA; q0 += 64) {
+        const int nq = min(64, nA - q0);
+        const int pq = cs + q0 + lane;
+        const bool qcore = lane < nq && (denseA || (S.flags[pq] & kFlagCore));
+        const uint64_t qm = __ballot(qcore);
+        if (qm == 0ull) continue;
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        int qslot = 0;
+        if (qcore) {
+            qx = S.sx[pq];
+            if (dim > 1) qy = S.sy[pq];
+            if (dim > 2) qz = S.sz[pq];
+            qslot = slot_of(S, pq);
+        }
+        for (int jb = 0; jb < TC; jb += 64) {
+            const int j = jb + lane;
+            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
+            const bool bcore = j < TC && (S.flags[pos] & kFlagCore);
+            const uint64_t bm = __ballot(bcore);
+            if (bm == 0ull) continue;
+            const float cx = S.sx[pos];
+            const float cy = dim > 1 ? S.sy[pos] : 0.f;
+            const float cz = dim > 2 ? S.sz[pos] : 0.f;
+            const int rep = bcore ? S.rep[pos] : 0;
+            uint64_t m = qm;
+            if (safeA) {
+                bool touched = false;
+                while (m) {
+                    const int qi = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    touched = touched ||
+                              (bcore && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
+                    if (__ballot(touched) == bm) break;  // every candidate core already touches A
+                }
+                unite_once_per_rep(S.parent, touched, rep, repA);
+            } else {
+                while (m) {
+                    const int qi = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const bool hit = bcore && pos < cs + q0 + qi &&
+                                     sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
+                    unite_once_per_rep(S.parent, hit, rep, lane_i(qslot, qi));
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
+    const int total = a.cell_pre[a.n];
+    const int nw = gridDim.x * kCellWaves;
+    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+        const CellCtx x = cell_of(a, gc);
+        union_cell(a.s[x.s], x.c);
     }
 }
 
@@ -1787,9 +1906,13 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         if (e != hipSuccess) return e;
     }
     PH_BEGIN(kPhUnion);
-    RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
-    RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
-    RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
+    if (sh.cellpar) {
+        RTDBS_LAUNCH(k_union, sh.cellpar_blocks, 64 * kCellWaves, a);
+    } else {
+        RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
+        RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
+        RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
+    }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
     RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
