@@ -209,6 +209,19 @@ int rtdbs_device_alloc(size_t bytes, void **ptr);
 int rtdbs_device_free(void *ptr);
 int rtdbs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
 int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+/* On-device brute-force DBSCAN of one window of n points (slot i = point i):
+ * every pair is tested, with the slide step's FP32 predicate and output rules
+ * (label 0 core / 1 border / 2 noise; cid = minimum core slot of the cluster,
+ * a border's = the minimum cluster id among cores within eps, noise = own
+ * slot); nn (optional) receives the self-inclusive neighbour counts. A
+ * validator for windows too large for a CPU check. Synchronous; allocates
+ * 32 bytes of device memory per point.
+ * Replaces: cluster_with_cuda (util.cpp:357-381) and its kernels
+ * find_neighbors_t / find_cores_t / set_cluster_id_op_t / post_cluster_t
+ * (aabb.cu:150-261), the reference's own check path. */
+int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const float *points, int64_t n, int32_t *label,
+                     int32_t *cid, int32_t *nn);
+
 /* Device-to-device copy by a plain kernel (bytes a multiple of 16): a kernel
  * of known traffic for scaling profiler byte counters. Synchronous. */
 int rtdbs_device_copy(void *dst_dev, const void *src_dev, size_t bytes);

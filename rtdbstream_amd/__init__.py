@@ -116,6 +116,8 @@ SIGNATURES = [
     ("rtdbs_memcpy_h2d", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_memcpy_d2h", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_device_copy", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("rtdbs_bruteforce", ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P, ctypes.c_int64, _P, _P,
+                                        _P]),
     ("rtdbs_device_synchronize", ctypes.c_int, []),
     ("rtdbs_device_count", ctypes.c_int, [_I32P]),
     ("rtdbs_batch_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
@@ -185,6 +187,22 @@ def autotune(dim, window, stride, eps, min_pts, points, slides=3, lo=None, hi=No
     _check(lib().rtdbs_autotune(ctypes.byref(p), _ptr(pts), pts.size // int(dim), int(slides), ctypes.byref(r)))
     return {"cell_div": r.cell_div, "lanes_per_query": r.lanes_per_query, "ms_per_slide": r.ms_per_slide,
             "from_cache": bool(r.from_cache), "trials": r.trials, "key": r.key.decode()}
+
+
+def bruteforce(points, eps, min_pts, dim=None):
+    """On-device brute-force DBSCAN of one window (every pair tested):
+    returns (label, cid, nn) with the slide step's rules, slot i = point i."""
+    pts = np.asarray(points, dtype=np.float32)
+    if dim is None:
+        dim = 1 if pts.ndim == 1 else pts.shape[1]
+    pts = np.ascontiguousarray(pts.reshape(-1), dtype=np.float32)
+    n = pts.size // int(dim) if int(dim) > 0 else 0
+    label = np.empty(max(n, 1), np.int32)
+    cid = np.empty(max(n, 1), np.int32)
+    nn = np.empty(max(n, 1), np.int32)
+    _check(lib().rtdbs_bruteforce(int(dim), float(eps), int(min_pts), _ptr(pts), n, _ptr(label), _ptr(cid),
+                                  _ptr(nn)))
+    return label, cid, nn
 
 
 def _f32(a, n, dim, what):

@@ -14,7 +14,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "librtdbs.so")
 BUILD_DIR = os.path.join(PKG, "_build")
 
-SOURCES = ["rtdbs_kernels.hip", "rtdbs_engine.cpp", "rtdbs_server.cpp", "rtdbs_tune.cpp"]
+SOURCES = ["rtdbs_kernels.hip", "rtdbs_engine.cpp", "rtdbs_server.cpp", "rtdbs_tune.cpp", "rtdbs_validate.hip"]
 
 # -ffp-contract=off: distances must be bit-identical to the FP32 reference
 # order (no fused multiply-add). Packed FP32 ops are disabled for all HIP code.

@@ -1,0 +1,78 @@
+// Device helpers shared by the slide kernels and the brute-force validator.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtdbs {
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// FP32 squared distance in the reference's order, never contracted:
+// util.cpp:310-313 / dbscan.cu:48-49 compute O.x*O.x + O.y*O.y + O.z*O.z.
+__device__ __forceinline__ float sqd(int dim, float ax, float ay, float az, float bx, float by,
+                                     float bz) {
+    float dx = __fsub_rn(ax, bx);
+    float d = __fmul_rn(dx, dx);
+    if (dim > 1) {
+        float dy = __fsub_rn(ay, by);
+        d = __fadd_rn(d, __fmul_rn(dy, dy));
+    }
+    if (dim > 2) {
+        float dz = __fsub_rn(az, bz);
+        d = __fadd_rn(d, __fmul_rn(dz, dz));
+    }
+    return d;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Union-find parent reads may be stale (served by a CU's L1 after another CU
+// lowered an entry): a stale value is still an ancestor, and hooks go through
+// L2 atomics, so stale reads cost extra steps, never correctness. Cacheable
+// loads keep the hot root entries of large clusters from funnelling every walk
+// into one L2 line.
+__device__ __forceinline__ int uf_load(const int32_t *p) { return *p; }
+
+// Union-find over ring slots, parent[x] <= x for every slot, so the root of a
+// component is its minimum slot - the reference's canonical cluster id (it
+// hooks the larger root under the smaller, dbscan.cu:22-43).
+//
+// Hooking uses atomicMin, which never fails: when b turned out not to be a
+// root any more, min(old, a) is stored and the caller goes on uniting a with
+// the displaced parent `old`, so no link is lost and b strictly decreases.
+// (A compare-and-swap hook fails and retries whenever another lane hooked the
+// same root first, which serialises every edge of a large cluster on its
+// root.) Path halving also lowers entries with atomicMin, so every entry only
+// ever decreases and always points at an ancestor.
+__device__ __forceinline__ int uf_find(int32_t *parent, int x) {
+    int p = uf_load(parent + x);
+    while (p != x) {
+        const int gp = uf_load(parent + p);
+        if (gp != p) atomicMin(parent + x, gp);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void uf_unite(int32_t *parent, int a, int b) {
+    while (true) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicMin(parent + b, a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+}  // namespace rtdbs

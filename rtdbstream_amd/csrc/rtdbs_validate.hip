@@ -1,0 +1,178 @@
+// On-device brute-force DBSCAN of one window: every pair of points is tested.
+// It is the large-window validator of the slide step. It plays the role of the
+// reference's own GPU check path: cluster_with_cuda and find_neighbors_t /
+// find_cores_t / set_cluster_id_op_t / post_cluster_t (util.cpp:357-381,
+// aabb.cu:150-261).
+// It uses the same FP32 distance predicate as the slide kernels (sqd: the
+// reference's un-contracted order, d2 < fl32(eps*eps)) and the same output
+// rules. It shares no grid, sorting or cell logic with them: its candidates
+// are all n points, staged through LDS in tiles.
+//   nn    - neighbours within eps, the point itself included (find_neighbors_t)
+//   label - 0 core (nn >= min_pts), 1 border, 2 noise (find_cores_t)
+//   cid   - cores: the minimum index of their cluster's cores (union-find
+//           hooking the larger root under the smaller, dbscan.cu:22-43, then
+//           post_cluster_t); borders: the minimum cluster id among the cores
+//           within eps; noise: its own index.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "rtdbs.h"
+#include "rtdbs_devfn.h"
+#include "rtdbs_host.h"
+
+namespace rtdbs {
+namespace {
+
+constexpr int kBfTile = 256;  // candidates staged per LDS tile = threads per block
+
+struct BfArgs {
+    const float *x, *y, *z;
+    int n, dim, min_pts;
+    float e2;
+    int32_t *nn, *parent, *label, *cid;
+};
+
+// Neighbour counts: one query per thread, candidates in LDS tiles.
+__global__ __launch_bounds__(kBfTile) void k_bf_count(BfArgs a) {
+    __shared__ float tx[kBfTile], ty[kBfTile], tz[kBfTile];
+    const int i = blockIdx.x * kBfTile + threadIdx.x;
+    const bool q = i < a.n;
+    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    int cnt = 0;
+    for (int t0 = 0; t0 < a.n; t0 += kBfTile) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        tx[threadIdx.x] = j < a.n ? a.x[j] : 0.f;
+        ty[threadIdx.x] = j < a.n ? a.y[j] : 0.f;
+        tz[threadIdx.x] = j < a.n ? a.z[j] : 0.f;
+        __syncthreads();
+        const int m = min(kBfTile, a.n - t0);
+        for (int k = 0; k < m; k++) cnt += sqd(a.dim, qx, qy, qz, tx[k], ty[k], tz[k]) < a.e2;
+    }
+    if (q) {
+        a.nn[i] = cnt;
+        a.label[i] = cnt >= a.min_pts ? 0 : 2;
+        a.parent[i] = i;
+    }
+}
+
+// Core-core unions: each core unites with every core j < i within eps, so
+// every pair is tested once.
+__global__ __launch_bounds__(kBfTile) void k_bf_union(BfArgs a) {
+    __shared__ float tx[kBfTile], ty[kBfTile], tz[kBfTile];
+    __shared__ int tc[kBfTile];
+    const int i = blockIdx.x * kBfTile + threadIdx.x;
+    const bool q = i < a.n && a.label[i] == 0;
+    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    const int last = min(a.n, (int)(blockIdx.x + 1) * kBfTile);  // candidates below the block's last query
+    for (int t0 = 0; t0 < last; t0 += kBfTile) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        const bool v = j < a.n;
+        tx[threadIdx.x] = v ? a.x[j] : 0.f;
+        ty[threadIdx.x] = v ? a.y[j] : 0.f;
+        tz[threadIdx.x] = v ? a.z[j] : 0.f;
+        tc[threadIdx.x] = v && a.label[j] == 0;
+        __syncthreads();
+        if (!q) continue;
+        const int m = min(kBfTile, i - t0);
+        for (int k = 0; k < m; k++)
+            if (tc[k] && sqd(a.dim, qx, qy, qz, tx[k], ty[k], tz[k]) < a.e2) uf_unite(a.parent, i, t0 + k);
+    }
+}
+
+__global__ __launch_bounds__(kBfTile) void k_bf_flatten(BfArgs a) {
+    const int i = blockIdx.x * kBfTile + threadIdx.x;
+    if (i >= a.n) return;
+    a.cid[i] = a.label[i] == 0 ? uf_find(a.parent, i) : i;
+}
+
+// Borders: the minimum cluster id among the cores within eps.
+__global__ __launch_bounds__(kBfTile) void k_bf_border(BfArgs a) {
+    __shared__ float tx[kBfTile], ty[kBfTile], tz[kBfTile];
+    __shared__ int tcid[kBfTile];
+    const int i = blockIdx.x * kBfTile + threadIdx.x;
+    const bool q = i < a.n && a.label[i] != 0;
+    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    int best = INT_MAX;
+    for (int t0 = 0; t0 < a.n; t0 += kBfTile) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        const bool v = j < a.n;
+        tx[threadIdx.x] = v ? a.x[j] : 0.f;
+        ty[threadIdx.x] = v ? a.y[j] : 0.f;
+        tz[threadIdx.x] = v ? a.z[j] : 0.f;
+        tcid[threadIdx.x] = v && a.label[j] == 0 ? a.cid[j] : INT_MAX;
+        __syncthreads();
+        if (!q) continue;
+        const int m = min(kBfTile, a.n - t0);
+        for (int k = 0; k < m; k++)
+            if (tcid[k] < best && sqd(a.dim, qx, qy, qz, tx[k], ty[k], tz[k]) < a.e2) best = tcid[k];
+    }
+    if (q && best != INT_MAX) {
+        a.label[i] = 1;
+        a.cid[i] = best;
+    }
+}
+
+#define BFCHK(expr)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            rc = fail(RTDBS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+            goto done;                                                               \
+        }                                                                            \
+    } while (0)
+
+}  // namespace
+}  // namespace rtdbs
+
+using namespace rtdbs;
+
+extern "C" int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const float *points, int64_t n,
+                                int32_t *label, int32_t *cid, int32_t *nn) {
+    if (dim < 1 || dim > 3) return fail(RTDBS_EINVAL, "dim must be 1, 2 or 3 (got %d)", dim);
+    if (!(eps > 0.0f) || !(eps < INFINITY)) return fail(RTDBS_EINVAL, "eps must be finite and > 0");
+    if (min_pts < 1) return fail(RTDBS_EINVAL, "min_pts must be >= 1 (got %d)", min_pts);
+    if (n < 1 || n > (int64_t)INT_MAX - kBfTile) return fail(RTDBS_EINVAL, "n out of range (got %lld)", (long long)n);
+    if (!points || !label || !cid) return fail(RTDBS_EINVAL, "NULL argument");
+    const size_t N = (size_t)n;
+    std::vector<float> soa(3 * N, 0.0f);
+    for (size_t i = 0; i < N; i++)
+        for (int d = 0; d < dim; d++) soa[d * N + i] = points[i * dim + d];
+    int rc = RTDBS_OK;
+    char *mem = nullptr;
+    BfArgs a{};
+    const unsigned blocks = (unsigned)((N + kBfTile - 1) / kBfTile);
+    BFCHK(hipMalloc((void **)&mem, 3 * N * 4 + 5 * N * 4));
+    a.x = (const float *)mem;
+    a.y = a.x + N;
+    a.z = a.y + N;
+    a.nn = (int32_t *)(mem + 3 * N * 4);
+    a.parent = a.nn + N;
+    a.label = a.parent + N;
+    a.cid = a.label + N;
+    a.n = (int)n;
+    a.dim = dim;
+    a.min_pts = min_pts;
+    a.e2 = eps * eps;  // fl32(eps*eps), as the reference's radius2 (dbscan.cpp:41)
+    BFCHK(hipMemcpy(mem, soa.data(), 3 * N * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_bf_count, dim3(blocks), dim3(kBfTile), 0, 0, a);
+    BFCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_bf_union, dim3(blocks), dim3(kBfTile), 0, 0, a);
+    BFCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_bf_flatten, dim3(blocks), dim3(kBfTile), 0, 0, a);
+    BFCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_bf_border, dim3(blocks), dim3(kBfTile), 0, 0, a);
+    BFCHK(hipGetLastError());
+    BFCHK(hipMemcpy(label, a.label, N * 4, hipMemcpyDeviceToHost));
+    BFCHK(hipMemcpy(cid, a.cid, N * 4, hipMemcpyDeviceToHost));
+    if (nn) BFCHK(hipMemcpy(nn, a.nn, N * 4, hipMemcpyDeviceToHost));
+done:
+    if (mem) (void)hipFree(mem);
+    return rc;
+}
