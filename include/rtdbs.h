@@ -209,6 +209,35 @@ int rtdbs_device_alloc(size_t bytes, void **ptr);
 int rtdbs_device_free(void *ptr);
 int rtdbs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
 int rtdbs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+/* ---- datasets ------------------------------------------------------------
+ * The five on-disk formats of the reference's runner (util.cpp:12-254). */
+enum {
+    RTDBS_FMT_AUTO = 0,    /* from the file name, as the reference's main (dbscan.cpp:2454-2469) */
+    RTDBS_FMT_TAO = 1,     /* CSV "x,y,z", no header; 3-D              (read_data_from_tao) */
+    RTDBS_FMT_GEOLIFE = 2, /* raw little-endian double x,y,z records   (read_data_from_geolife) */
+    RTDBS_FMT_RBF = 3,     /* header line, CSV "x,y"; 2-D              (read_data_from_rbf) */
+    RTDBS_FMT_EDS = 4,     /* header line, "id x y cid"; 2-D           (read_data_from_eds) */
+    RTDBS_FMT_STOCK = 5    /* header line, one value per line; 1-D     (read_data_from_stk) */
+};
+
+typedef struct {
+    int32_t format;  /* RTDBS_FMT_* actually read */
+    int32_t dim;
+    int64_t n;       /* records */
+    float lo[3];     /* per-axis min / max of the stored FP32 values (the */
+    float hi[3];     /* reference's state.min_value / max_value) */
+    float *points;   /* n * dim floats, point-major; owned, rtdbs_dataset_free */
+} rtdbs_dataset;
+
+/* Format named by the file name (substrings tao, geolife, RBF, EDS, stock, in
+ * the reference's order), RTDBS_FMT_AUTO if none. */
+int rtdbs_dataset_format_of(const char *path);
+/* Reads the first n records (n <= 0: all) of `path`. Fails on a record that
+ * does not parse or when fewer than n records exist.
+ * Replaces: read_data_from_tao / _geolife / _rbf / _eds / _stk (util.cpp:12-254). */
+int rtdbs_dataset_load(const char *path, int32_t format, int64_t n, rtdbs_dataset *out);
+void rtdbs_dataset_free(rtdbs_dataset *ds);
+
 /* On-device brute-force DBSCAN of one window of n points (slot i = point i):
  * every pair is tested, with the slide step's FP32 predicate and output rules
  * (label 0 core / 1 border / 2 noise; cid = minimum core slot of the cluster,

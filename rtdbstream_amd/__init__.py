@@ -83,6 +83,12 @@ _I32P = ctypes.POINTER(ctypes.c_int32)
 _F32P = ctypes.POINTER(ctypes.c_float)
 
 # (name, restype, argtypes) of every exported entry point of include/rtdbs.h
+class Dataset(ctypes.Structure):
+    """rtdbs_dataset: a loaded dataset (points owned by the library)."""
+    _fields_ = [("format", ctypes.c_int32), ("dim", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3), ("points", ctypes.POINTER(ctypes.c_float))]
+
+
 SIGNATURES = [
     ("rtdbs_stream_create", ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_P)]),
     ("rtdbs_stream_device_bytes", ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_size_t)]),
@@ -116,6 +122,9 @@ SIGNATURES = [
     ("rtdbs_memcpy_h2d", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_memcpy_d2h", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("rtdbs_device_copy", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("rtdbs_dataset_format_of", ctypes.c_int, [ctypes.c_char_p]),
+    ("rtdbs_dataset_load", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Dataset)]),
+    ("rtdbs_dataset_free", None, [ctypes.POINTER(Dataset)]),
     ("rtdbs_bruteforce", ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P, ctypes.c_int64, _P, _P,
                                         _P]),
     ("rtdbs_device_synchronize", ctypes.c_int, []),

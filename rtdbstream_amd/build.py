@@ -12,9 +12,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "librtdbs.so")
+RUNNER = os.path.join(PKG, "rtdbs_run")  # command-line runner, links librtdbs.so
 BUILD_DIR = os.path.join(PKG, "_build")
 
-SOURCES = ["rtdbs_kernels.hip", "rtdbs_engine.cpp", "rtdbs_server.cpp", "rtdbs_tune.cpp", "rtdbs_validate.hip"]
+SOURCES = ["rtdbs_kernels.hip", "rtdbs_engine.cpp", "rtdbs_server.cpp", "rtdbs_tune.cpp", "rtdbs_validate.hip", "rtdbs_dataset.cpp"]
 
 # -ffp-contract=off: distances must be bit-identical to the FP32 reference
 # order (no fused multiply-add). Packed FP32 ops are disabled for all HIP code.
@@ -61,6 +62,12 @@ def build_library(verbose=False):
     tmp = LIB + ".tmp"
     outs.append(_run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]))
     os.replace(tmp, LIB)
+    # the runner: host code only, linked against the library next to it
+    robj = os.path.join(BUILD_DIR, "rtdbs_run.o")
+    outs.append(_run([cc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "rtdbs_run.cpp"), "-o", robj]))
+    rtmp = RUNNER + ".tmp"
+    outs.append(_run([cc, "-o", rtmp, robj, "-L" + PKG, "-lrtdbs", "-Wl,-rpath,$ORIGIN"]))
+    os.replace(rtmp, RUNNER)
     if verbose:
         for o in outs:
             if o.strip():
