@@ -258,6 +258,34 @@ int rtdbs_device_synchronize(void);
 int rtdbs_device_count(int *count);
 
 /* ---- kernel timing (HIP events on the engine's own streams) -------------- */
+/* Work counters of one stream, summed over the slides since counting was
+ * enabled - the analogue of the reference's DEBUG_INFO statistics
+ * (print_rt_info / print_overall_rt_info, dbscan.cpp:145-212). */
+typedef struct {
+    int64_t slides;        /* slides counted */
+    int64_t cells;         /* non-empty grid cells (the reference's spheres, num_spheres) */
+    int64_t dense_points;  /* points of dense cells: core without counting (num_points_in_dense_cells) */
+    int64_t count_queries; /* points whose neighbours were counted (rays of the core pass) */
+    int64_t count_tests;   /* distance tests of the range count (intersections; num_dist_calculations) */
+    int64_t merge_tests;   /* distance tests of the cluster merge (cluster-pass intersections) */
+    int64_t border_tests;  /* distance tests of the border pass */
+    int64_t unites;        /* union-find unite calls */
+    int64_t merge_pairs;   /* touching element pairs found by the merge */
+    int32_t cell_parallel; /* 1: counted on the cell-parallel kernels; 0: point-parallel, where
+                              merge_tests, unites and merge_pairs are not counted */
+    int32_t pad;
+    int64_t reserved[6];
+} rtdbs_stats;
+
+/* Enables (1) or disables (0) the counters of every stream of the batch and
+ * zeroes them. Counting adds one atomic per counter per wave, so it is off by
+ * default and timings taken with it on are not representative. */
+int rtdbs_batch_set_stats(rtdbs_batch *b, int enable);
+/* Counters of member `index` of the batch (synchronises). */
+int rtdbs_batch_get_stats(rtdbs_batch *b, int index, rtdbs_stats *out);
+int rtdbs_stream_set_stats(rtdbs_stream *s, int enable);
+int rtdbs_stream_get_stats(rtdbs_stream *s, rtdbs_stats *out);
+
 /* Event timing on a batch (or a stream's solo batch): enable = 0 off,
  * 1 every phase, 2 only the range-count kernel (two events per slide). */
 int rtdbs_batch_set_timing(rtdbs_batch *b, int enable);

@@ -83,6 +83,17 @@ _I32P = ctypes.POINTER(ctypes.c_int32)
 _F32P = ctypes.POINTER(ctypes.c_float)
 
 # (name, restype, argtypes) of every exported entry point of include/rtdbs.h
+class Stats(ctypes.Structure):
+    """rtdbs_stats: per-stream work counters summed over counted slides."""
+    _fields_ = [("slides", ctypes.c_int64), ("cells", ctypes.c_int64), ("dense_points", ctypes.c_int64),
+                ("count_queries", ctypes.c_int64), ("count_tests", ctypes.c_int64), ("merge_tests", ctypes.c_int64),
+                ("border_tests", ctypes.c_int64), ("unites", ctypes.c_int64), ("merge_pairs", ctypes.c_int64),
+                ("cell_parallel", ctypes.c_int32), ("pad", ctypes.c_int32), ("reserved", ctypes.c_int64 * 6)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "reserved")}
+
+
 class Dataset(ctypes.Structure):
     """rtdbs_dataset: a loaded dataset (points owned by the library)."""
     _fields_ = [("format", ctypes.c_int32), ("dim", ctypes.c_int32), ("n", ctypes.c_int64),
@@ -129,6 +140,10 @@ SIGNATURES = [
                                         _P]),
     ("rtdbs_device_synchronize", ctypes.c_int, []),
     ("rtdbs_device_count", ctypes.c_int, [_I32P]),
+    ("rtdbs_batch_set_stats", ctypes.c_int, [_P, ctypes.c_int]),
+    ("rtdbs_batch_get_stats", ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Stats)]),
+    ("rtdbs_stream_set_stats", ctypes.c_int, [_P, ctypes.c_int]),
+    ("rtdbs_stream_get_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("rtdbs_batch_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("rtdbs_stream_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("rtdbs_batch_get_timing", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
@@ -316,6 +331,15 @@ class Stream:
     def set_timing(self, enable=True):
         _check(lib().rtdbs_stream_set_timing(self.h, int(enable)))
 
+    def set_stats(self, enable=True):
+        """Turns the work counters on (zeroed) or off."""
+        _check(lib().rtdbs_stream_set_stats(self.h, int(enable)))
+
+    def stats(self):
+        st = Stats()
+        _check(lib().rtdbs_stream_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
     def timing(self):
         return _get_timing(lib().rtdbs_stream_get_timing, self.h)
 
@@ -375,6 +399,14 @@ class Batch:
 
     def set_timing(self, enable=True):
         _check(lib().rtdbs_batch_set_timing(self.h, int(enable)))
+
+    def set_stats(self, enable=True):
+        _check(lib().rtdbs_batch_set_stats(self.h, int(enable)))
+
+    def stats(self, index):
+        st = Stats()
+        _check(lib().rtdbs_batch_get_stats(self.h, int(index), ctypes.byref(st)))
+        return st.as_dict()
 
     def timing(self):
         return _get_timing(lib().rtdbs_batch_get_timing, self.h)

@@ -23,7 +23,18 @@ constexpr int kMaxSlotBits = 24;
 constexpr uint32_t kEpochMax = (1u << 24) - 1;  // hash epochs cycle 1..kEpochMax
 constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
-constexpr int kNumCounters = 8;
+// Per-stream work counters (optional; rtdbs_stats in the C ABI)
+constexpr int kNumStats = 8;
+enum Stat {
+    kStatCells = 0,      // non-empty cells
+    kStatDensePoints,    // points of dense cells: core without counting
+    kStatQueries,        // points whose neighbours were counted
+    kStatCountTests,     // distance tests of the range count
+    kStatMergeTests,     // distance tests of the cluster merge
+    kStatBorderTests,    // distance tests of the border pass
+    kStatUnites,         // union-find unite calls
+    kStatMergePairs      // touching (element, element) pairs found by the merge
+};
 
 constexpr uint8_t kFlagCore = 1;
 constexpr uint8_t kFlagDense = 2;
@@ -97,7 +108,7 @@ struct BatchArgs {
     const int32_t *pre_wtile;  // n+1 prefix of ceil(W/kWTile)
     const int32_t *pre_itile;  // n+1 prefix of ceil(W/kTile) (first-window sort)
     const int32_t *pre_chunk;  // n+1 prefix of ceil(W/kChunk)
-    int32_t *counters;         // kNumCounters work counters
+    unsigned long long *stats; // n * kNumStats per-stream counters, null when off
     int32_t *cell_pre;         // n+1 prefix of cells per stream (written by k_rows)
 };
 

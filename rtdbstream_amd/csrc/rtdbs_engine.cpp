@@ -482,14 +482,14 @@ static int batch_alloc(rtdbs_batch *b) {
     const size_t desc_b = ((size_t)n * sizeof(StreamDev) + 255) & ~size_t(255);
     const size_t io_b = ((size_t)n * sizeof(IO) + 255) & ~size_t(255);
     const size_t pre_b = ((size_t)4 * (n + 1) * 4 + 255) & ~size_t(255);
-    const size_t cnt_b = 256;
+    const size_t cnt_b = ((size_t)n * kNumStats * 8 + 255) & ~size_t(255);
     const size_t cpre_b = ((size_t)(n + 1) * 4 + 255) & ~size_t(255);
     HIPCHK(hipMalloc((void **)&b->dmem, desc_b + io_b + pre_b + cnt_b + cpre_b));
     b->d_cell_pre = (int32_t *)(b->dmem + desc_b + io_b + pre_b + cnt_b);
     b->d_desc = (StreamDev *)b->dmem;
     b->d_io = (IO *)(b->dmem + desc_b);
     b->d_pre = (int32_t *)(b->dmem + desc_b + io_b);
-    b->d_counters = (int32_t *)(b->dmem + desc_b + io_b + pre_b);
+    b->d_stats = (unsigned long long *)(b->dmem + desc_b + io_b + pre_b);
     HIPCHK(hipHostMalloc((void **)&b->h_io, (size_t)n * sizeof(IO), hipHostMallocDefault));
     HIPCHK(hipStreamCreateWithFlags(&b->hs, hipStreamNonBlocking));
     for (int i = 0; i < 2 * kNumPhases; i++) HIPCHK(hipEventCreate(&b->ev[i]));
@@ -615,7 +615,7 @@ int refresh_batch(rtdbs_batch *b) {
     a.pre_wtile = b->d_pre + (n + 1);
     a.pre_chunk = b->d_pre + 2 * (n + 1);
     a.pre_itile = b->d_pre + 3 * (n + 1);
-    a.counters = b->d_counters;
+    a.stats = b->stats_on ? b->d_stats : nullptr;
     a.cell_pre = b->d_cell_pre;
     if (b->gexec) {
         (void)hipGraphExecDestroy(b->gexec);
@@ -688,6 +688,7 @@ int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *
     HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr, b->timing_mask));
     b->pending_timing = b->timing;
     for (auto *s : b->streams) s->slides++;
+    if (b->stats_on) b->stats_slides++;
     return RTDBS_OK;
 }
 
@@ -758,6 +759,7 @@ int batch_launch(rtdbs_batch *b, const IO *io) {
         b->pending_timing = b->timing;
     }
     for (auto *s : b->streams) s->slides++;
+    if (b->stats_on) b->stats_slides++;
     return RTDBS_OK;
 }
 
@@ -823,6 +825,64 @@ extern "C" int rtdbs_batch_sync(rtdbs_batch *b) {
 }
 
 extern "C" void rtdbs_batch_destroy(rtdbs_batch *b) { batch_free(b); }
+
+// ---------------------------------------------------------------------------
+// work counters
+
+extern "C" int rtdbs_batch_set_stats(rtdbs_batch *b, int enable) {
+    if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    HIPCHK(hipMemset(b->d_stats, 0, (size_t)b->capacity * kNumStats * 8));
+    b->stats_on = enable != 0;
+    b->stats_slides = 0;
+    b->args.stats = b->stats_on ? b->d_stats : nullptr;
+    // a captured graph holds the previous argument
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_batch_get_stats(rtdbs_batch *b, int index, rtdbs_stats *out) {
+    if (!b || !out) return fail(RTDBS_EINVAL, "NULL argument");
+    if (index < 0 || index >= b->n) return fail(RTDBS_EINVAL, "stream index %d outside [0, %d)", index, b->n);
+    int rc = batch_sync(b);
+    if (rc) return rc;
+    unsigned long long v[kNumStats];
+    HIPCHK(hipMemcpy(v, b->d_stats + (size_t)index * kNumStats, sizeof(v), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->slides = b->stats_slides;
+    out->cells = (int64_t)v[kStatCells];
+    out->dense_points = (int64_t)v[kStatDensePoints];
+    out->count_queries = (int64_t)v[kStatQueries];
+    out->count_tests = (int64_t)v[kStatCountTests];
+    out->merge_tests = (int64_t)v[kStatMergeTests];
+    out->border_tests = (int64_t)v[kStatBorderTests];
+    out->unites = (int64_t)v[kStatUnites];
+    out->merge_pairs = (int64_t)v[kStatMergePairs];
+    out->cell_parallel = b->shape.cellpar;
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_stream_set_stats(rtdbs_stream *s, int enable) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    int rc = ensure_solo(s);
+    if (rc) return rc;
+    return rtdbs_batch_set_stats(s->solo, enable);
+}
+
+extern "C" int rtdbs_stream_get_stats(rtdbs_stream *s, rtdbs_stats *out) {
+    if (!s) return fail(RTDBS_EINVAL, "stream is NULL");
+    int rc = ensure_solo(s);
+    if (rc) return rc;
+    return rtdbs_batch_get_stats(s->solo, 0, out);
+}
 
 // ---------------------------------------------------------------------------
 // timing

@@ -43,7 +43,9 @@ struct rtdbs_batch {
     rtdbs::StreamDev *d_desc = nullptr;
     rtdbs::IO *d_io = nullptr;
     int32_t *d_pre = nullptr;
-    int32_t *d_counters = nullptr;
+    unsigned long long *d_stats = nullptr;  // capacity * kNumStats work counters
+    bool stats_on = false;
+    int64_t stats_slides = 0;
     int32_t *d_cell_pre = nullptr;
     rtdbs::IO *h_io = nullptr;  // pinned mirror of d_io
     bool io_valid = false;

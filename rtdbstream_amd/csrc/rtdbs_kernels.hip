@@ -329,8 +329,16 @@ __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds
     const int tile = g - a.pre_wtile[s];
     const int p0 = tile * kWTile, p1 = min(p0 + kWTile, S.W);
     const int lo_slot = S.mut->u * S.S;
+    uint64_t k[kWTile / kTileThreads];
+#pragma unroll
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
+        const int p = p0 + threadIdx.x + r * kTileThreads;
+        k[r] = p < p1 ? S.skey[p] : 0ull;
+    }
     int cnt = 0;
-    for (int p = p0 + threadIdx.x; p < p1; p += kTileThreads) cnt += survives(S, S.skey[p], lo_slot);
+#pragma unroll
+    for (int r = 0; r < kWTile / kTileThreads; r++)
+        cnt += (p0 + (int)threadIdx.x + r * kTileThreads < p1) && survives(S, k[r], lo_slot);
     cnt = block_sum(cnt, lds4);
     if (threadIdx.x == 0) S.tile_a[tile] = cnt;
 }
@@ -343,7 +351,6 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
     const int nst = a.pre_stile[a.n];
-    if (g == 0 && threadIdx.x < kNumCounters) a.counters[threadIdx.x] = 0;
     if (g >= nst) {
         expire_count(a, g - nst, (int *)lds);
         return;
@@ -381,26 +388,35 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
     const int tile = g - a.pre_wtile[s];
     const int p0 = tile * kWTile;
     const int lo_slot = S.mut->u * S.S;
+    constexpr int per = kWTile / kTileThreads;
+    const int dim = S.dim;
+    // every load of the tile issued before the scan and the stores
+    uint64_t key[per];
+    float cx[per], cy[per], cz[per];
+#pragma unroll
+    for (int r = 0; r < per; r++) {
+        const int p = p0 + r * kTileThreads + threadIdx.x;
+        const int pp = p < S.W ? p : 0;
+        key[r] = S.skey[pp];
+        cx[r] = S.sx[pp];
+        cy[r] = dim > 1 ? S.sy[pp] : 0.f;
+        cz[r] = dim > 2 ? S.sz[pp] : 0.f;
+    }
     int part = 0;
     for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_a[t];
     int run = block_sum(part, lds4);
-    const int dim = S.dim;
-    for (int r = 0; r < kWTile / kTileThreads; r++) {
+#pragma unroll
+    for (int r = 0; r < per; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
-        uint64_t key = 0;
-        bool f = false;
-        if (p < S.W) {
-            key = S.skey[p];
-            f = survives(S, key, lo_slot);
-        }
+        const bool f = p < S.W && survives(S, key[r], lo_slot);
         int tot;
         const int rank = block_rank_flags(f, &tot, lds4);
         if (f) {
             const int d = run + rank;
-            S.akey[d] = key;
-            S.ax[d] = S.sx[p];
-            if (dim > 1) S.ay[d] = S.sy[p];
-            if (dim > 2) S.az[d] = S.sz[p];
+            S.akey[d] = key[r];
+            S.ax[d] = cx[r];
+            if (dim > 1) S.ay[d] = cy[r];
+            if (dim > 2) S.az[d] = cz[r];
         }
         run += tot;
     }
@@ -410,9 +426,14 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
 // sorted tile at its rank in the whole stride (its index in the tile plus,
 // for each other tile, the number of that tile's keys below it) and write
 // keys and coordinates to the B1 arrays - the stride sorted by (cell, slot);
-// the remaining blocks compact the window's survivors.
+// the remaining blocks compact the window's survivors. The other tiles are
+// staged kRankGroup at a time, so a stride of up to kRankGroup + 1 tiles
+// costs one load round trip, and the binary searches of all staged tiles run
+// interleaved.
+constexpr int kRankGroup = 4;
+
 __global__ __launch_bounds__(kTileThreads) void k_rank_compact(BatchArgs a) {
-    __shared__ uint64_t lds[kTile];
+    __shared__ uint64_t lds[kRankGroup * kTile];
     const int g = blockIdx.x;
     const int nst = a.pre_stile[a.n];
     if (g >= nst) {
@@ -433,28 +454,71 @@ __global__ __launch_bounds__(kTileThreads) void k_rank_compact(BatchArgs a) {
         key[e] = i < nvalid ? S.bkey0[tile0 + i] : ~0ull;
         rank[e] = i;
     }
-    for (int t = 0; t < nt; t++) {
-        if (t == tile) continue;
-        const int n2 = min(kTile, S.S - t * kTile);
-        __syncthreads();
-        for (int i = threadIdx.x; i < n2; i += kTileThreads) lds[i] = S.bkey0[t * kTile + i];
+    // other tiles of the stream, kRankGroup per round
+    for (int o0 = 0; o0 < nt - 1; o0 += kRankGroup) {
+        int n2[kRankGroup];
+        uint64_t stage[kRankGroup][8];  // every load issued before any LDS store
+#pragma unroll
+        for (int q = 0; q < kRankGroup; q++) {
+            const int o = o0 + q;
+            const int t = o < tile ? o : o + 1;  // the o-th tile other than this one
+            n2[q] = o < nt - 1 ? min(kTile, S.S - t * kTile) : 0;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int i = threadIdx.x + e * kTileThreads;
+                stage[q][e] = i < n2[q] ? S.bkey0[t * kTile + i] : ~0ull;
+            }
+        }
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 8; e++) rank[e] += lds_lower_bound(lds, n2, key[e]);
+        for (int q = 0; q < kRankGroup; q++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) lds[q * kTile + threadIdx.x + e * kTileThreads] = stage[q][e];
+        __syncthreads();
+        // branch-free binary searches of all (tile, key) pairs, advanced one
+        // step together so their LDS reads overlap
+        int pos[kRankGroup][8];
+#pragma unroll
+        for (int q = 0; q < kRankGroup; q++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) pos[q][e] = 0;
+#pragma unroll
+        for (int step = kTile; step > 0; step >>= 1) {
+#pragma unroll
+            for (int q = 0; q < kRankGroup; q++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int probe = pos[q][e] + step;
+                    const uint64_t v = lds[q * kTile + min(probe, kTile) - 1];
+                    if (probe <= n2[q] && v < key[e]) pos[q][e] = probe;
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < kRankGroup; q++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) rank[e] += pos[q][e];
     }
     const uint32_t m = slot_mask(S);
     const int slot0 = S.mut->u * S.S, dim = S.dim;
+    float px[8], py[8], pz[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = threadIdx.x + e * kTileThreads;
+        const int gi = i < nvalid ? (int)((uint32_t)key[e] & m) - slot0 : 0;
+        const float *p = io.in + (size_t)gi * dim;
+        px[e] = p[0];
+        py[e] = dim > 1 ? p[1] : 0.f;
+        pz[e] = dim > 2 ? p[2] : 0.f;
+    }
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int i = threadIdx.x + e * kTileThreads;
         if (i < nvalid) {
             const int r = rank[e];
-            const int gi = (int)((uint32_t)key[e] & m) - slot0;
-            const float *p = io.in + (size_t)gi * dim;
             S.bkey1[r] = key[e];
-            S.bx1[r] = p[0];
-            if (dim > 1) S.by1[r] = p[1];
-            if (dim > 2) S.bz1[r] = p[2];
+            S.bx1[r] = px[e];
+            if (dim > 1) S.by1[r] = py[e];
+            if (dim > 2) S.bz1[r] = pz[e];
         }
     }
 }
@@ -507,10 +571,18 @@ __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint
     const int a0 = m.sp[0], a1 = m.sp[1];
     const int b0 = d0 - a0, b1 = d1 - a1;
     const int na = a1 - a0, nb = b1 - b0, n = na + nb;
-    for (int i = tid; i < na; i += kTileThreads) m.in[i] = A[a0 + i];
-    for (int i = tid; i < nb; i += kTileThreads) m.in[na + i] = B[b0 + i];
-    __syncthreads();
     constexpr int per = T / kTileThreads;
+    {
+        uint64_t v[per];  // every load issued before the LDS stores
+#pragma unroll
+        for (int r = 0; r < per; r++) {
+            const int j = tid + r * kTileThreads;
+            v[r] = j < na ? A[a0 + j] : (j < n ? B[b0 + j - na] : 0ull);
+        }
+#pragma unroll
+        for (int r = 0; r < per; r++) m.in[tid + r * kTileThreads] = v[r];
+    }
+    __syncthreads();
     const int diag = tid * per;
     if (diag < n) {
         int lo = max(0, diag - nb), hi = min(diag, na);
@@ -595,23 +667,32 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
     }
     __syncthreads();
     const int dim = S.dim, sb = S.slot_bits;
+    constexpr int per = kWTile / kTileThreads;
+    float cx[per], cy[per], cz[per];  // every gather issued before the stores
+#pragma unroll
+    for (int r = 0; r < per; r++) {
+        const int i = threadIdx.x + r * kTileThreads;
+        const int sidx = i < o1 - o0 ? m.src[i] : 0;
+        const bool fromA = sidx >= 0;
+        const int j = fromA ? sidx : ~sidx;
+        const float *px = fromA ? S.ax : S.bx1, *py = fromA ? S.ay : S.by1, *pz = fromA ? S.az : S.bz1;
+        cx[r] = px[j];
+        cy[r] = dim > 1 ? py[j] : 0.f;
+        cz[r] = dim > 2 ? pz[j] : 0.f;
+    }
     int starts = 0;
-    for (int i = threadIdx.x; i < o1 - o0; i += kTileThreads) {
-        const uint64_t key = m.out[i];
-        const int sidx = m.src[i];
-        S.skey[o0 + i] = key;
-        if (sidx >= 0) {
-            S.sx[o0 + i] = S.ax[sidx];
-            if (dim > 1) S.sy[o0 + i] = S.ay[sidx];
-            if (dim > 2) S.sz[o0 + i] = S.az[sidx];
-        } else {
-            const int j = ~sidx;
-            S.sx[o0 + i] = S.bx1[j];
-            if (dim > 1) S.sy[o0 + i] = S.by1[j];
-            if (dim > 2) S.sz[o0 + i] = S.bz1[j];
+#pragma unroll
+    for (int r = 0; r < per; r++) {
+        const int i = threadIdx.x + r * kTileThreads;
+        if (i < o1 - o0) {
+            const uint64_t key = m.out[i];
+            S.skey[o0 + i] = key;
+            S.sx[o0 + i] = cx[r];
+            if (dim > 1) S.sy[o0 + i] = cy[r];
+            if (dim > 2) S.sz[o0 + i] = cz[r];
+            const uint64_t prev = i > 0 ? m.out[i - 1] : m.pred;
+            starts += (o0 + i == 0) || ((key >> sb) != (prev >> sb));
         }
-        const uint64_t prev = i > 0 ? m.out[i - 1] : m.pred;
-        starts += (o0 + i == 0) || ((key >> sb) != (prev >> sb));
     }
     starts = block_sum(starts, lds4);
     if (threadIdx.x == 0) S.tile_b[tile] = starts;
@@ -633,18 +714,27 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     const int p0 = tile * kWTile;
     const uint32_t epoch = S.mut->epoch;
     const int sb = S.slot_bits;
+    constexpr int per = kWTile / kTileThreads;
+    uint64_t kc[per], kp[per];  // keys of each position and its predecessor, loaded up front
+#pragma unroll
+    for (int r = 0; r < per; r++) {
+        const int p = p0 + r * kTileThreads + threadIdx.x;
+        kc[r] = p < S.W ? S.skey[p] : 0ull;
+        kp[r] = (p < S.W && p > 0) ? S.skey[p - 1] : 0ull;
+    }
     int part = 0;
     for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_b[t];
     const int base = block_sum(part, lds4);
     int run = 0;
-    for (int r = 0; r < kWTile / kTileThreads; r++) {
+#pragma unroll
+    for (int r = 0; r < per; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
         const bool valid = p < S.W;
         uint64_t ck = 0;
         bool f = false;
         if (valid) {
-            ck = S.skey[p] >> sb;
-            f = (p == 0) || (ck != (S.skey[p - 1] >> sb));
+            ck = kc[r] >> sb;
+            f = (p == 0) || (ck != (kp[r] >> sb));
         }
         int tot;
         const int rank = block_rank_flags(f, &tot, lds4);
@@ -920,6 +1010,28 @@ __device__ __forceinline__ float lane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// Work counters of one wave (uniform values), flushed to the per-stream
+// totals with one atomic per counter when the wave moves to another stream
+// or ends. Only when counting is on (a.stats non-null).
+struct WaveStats {
+    int s = -1;
+    unsigned long long v[kNumStats] = {};
+};
+
+__device__ __forceinline__ void stats_flush(const BatchArgs &a, WaveStats &w) {
+    if (a.stats && w.s >= 0 && lane_id() == 0)
+        for (int k = 0; k < kNumStats; k++)
+            if (w.v[k]) atomicAdd(a.stats + (size_t)w.s * kNumStats + k, w.v[k]);
+    for (int k = 0; k < kNumStats; k++) w.v[k] = 0;
+}
+
+__device__ __forceinline__ void stats_stream(const BatchArgs &a, WaveStats &w, int s) {
+    if (s != w.s) {
+        stats_flush(a, w);
+        w.s = s;
+    }
+}
+
 // The cell's stencil rows, one per lane (lane < R): start position `rx` and
 // exclusive prefix `pre` of the row lengths; returns the candidate total.
 __device__ __forceinline__ int cell_rows(const StreamDev &S, int A, int *rx, int *pre) {
@@ -952,11 +1064,12 @@ __device__ __forceinline__ int cell_candidate(int rx, int pre, int R, int j) {
 // aabb.cu:575-587). Dense cells are core without counting (all pairs within
 // eps). Otherwise each query counts its neighbours, itself included, and
 // stops once the count reaches min_pts.
-__device__ void count_cell(const StreamDev &S, int A) {
+__device__ void count_cell(const StreamDev &S, int A, WaveStats &st) {
     const int lane = lane_id();
     const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
     const uint8_t cf = S.cflag[A];
     if (cf & kCellDense) {
+        st.v[kStatDensePoints] += nA;
         for (int i = lane; i < nA; i += 64) {
             const int p = cs + i, sl = slot_of(S, p);
             S.parent[sl] = sl;
@@ -972,6 +1085,7 @@ __device__ void count_cell(const StreamDev &S, int A) {
     int rx, pre;
     const int TC = cell_rows(S, A, &rx, &pre);
     int fc = kNoCore;
+    st.v[kStatQueries] += nA;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + lane;
@@ -991,6 +1105,7 @@ __device__ void count_cell(const StreamDev &S, int A) {
             const float cz = dim > 2 ? S.sz[pos] : 0.f;
             const bool v = j < TC;
             uint64_t m = open;
+            st.v[kStatCountTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, TC - jb);
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1019,16 +1134,19 @@ __device__ void count_cell(const StreamDev &S, int A) {
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
     const int total = a.cell_pre[a.n];
     const int nw = gridDim.x * kCellWaves;
+    WaveStats st;
     for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        count_cell(a.s[x.s], x.c);
+        stats_stream(a, st, x.s);
+        count_cell(a.s[x.s], x.c, st);
     }
+    stats_flush(a, st);
 }
 
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
 // deterministic rule: the minimum canonical id among cores within eps) for
 // the non-core points of a cell; cells without them are skipped.
-__device__ void border_cell(const StreamDev &S, const IO &io, int A) {
+__device__ void border_cell(const StreamDev &S, const IO &io, int A, WaveStats &st) {
     const int lane = lane_id();
     if (S.cflag[A] & kCellDense) return;
     const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
@@ -1058,6 +1176,7 @@ __device__ void border_cell(const StreamDev &S, const IO &io, int A) {
             const bool core = j < TC && (S.flags[pos] & kFlagCore);
             const int cid = core ? S.canon[pos] : INT_MAX;
             uint64_t m = open;
+            st.v[kStatBorderTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, TC - jb);
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1078,23 +1197,29 @@ __device__ void border_cell(const StreamDev &S, const IO &io, int A) {
 __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
     const int total = a.cell_pre[a.n];
     const int nw = gridDim.x * kCellWaves;
+    WaveStats st;
     for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        border_cell(a.s[x.s], a.use_io1 ? a.io1 : a.io[x.s], x.c);
+        stats_stream(a, st, x.s);
+        border_cell(a.s[x.s], a.use_io1 ? a.io1 : a.io[x.s], x.c, st);
     }
+    stats_flush(a, st);
 }
 
 // Unites `target` with `rep` for every lane that wants it, once per run of
 // lanes holding the same representative (a safe cell's points are adjacent
 // in a candidate chunk and share one representative).
-__device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, int rep, int target) {
+__device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, int rep, int target, WaveStats &st) {
     const int lane = lane_id();
     const uint64_t m = __ballot(want);
     if (m == 0ull) return;
     const uint64_t below = m & ((1ull << lane) - 1ull);
     const int prev = below ? 63 - __clzll((long long)below) : lane;
     const int prep = __shfl(rep, prev);
-    if (want && (below == 0ull || prep != rep)) uf_unite(parent, target, rep);
+    const bool first = want && (below == 0ull || prep != rep);
+    st.v[kStatMergePairs] += __popcll(m);
+    st.v[kStatUnites] += __popcll(__ballot(first));
+    if (first) uf_unite(parent, target, rep);
 }
 
 // Cluster merge (the reference's __raygen__cluster + unite,
@@ -1107,7 +1232,7 @@ __device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, i
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
-__device__ void union_cell(const StreamDev &S, int A) {
+__device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
     const int lane = lane_id();
     const int fcA = S.fcore[A];
     if (fcA == kNoCore) return;
@@ -1157,24 +1282,27 @@ __device__ void union_cell(const StreamDev &S, int A) {
             const float cy = dim > 1 ? S.sy[pos] : 0.f;
             const float cz = dim > 2 ? S.sz[pos] : 0.f;
             const int rep = bcore ? S.rep[pos] : 0;
+            const unsigned chunk = (unsigned)min(64, TC - jb);
             uint64_t m = qm;
             if (safeA) {
                 bool touched = false;
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
+                    st.v[kStatMergeTests] += chunk;
                     touched = touched ||
                               (bcore && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                     if (__ballot(touched) == bm) break;  // every candidate core already touches A
                 }
-                unite_once_per_rep(S.parent, touched, rep, repA);
+                unite_once_per_rep(S.parent, touched, rep, repA, st);
             } else {
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
+                    st.v[kStatMergeTests] += chunk;
                     const bool hit = bcore && pos < cs + q0 + qi &&
                                      sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
-                    unite_once_per_rep(S.parent, hit, rep, lane_i(qslot, qi));
+                    unite_once_per_rep(S.parent, hit, rep, lane_i(qslot, qi), st);
                 }
             }
         }
@@ -1184,10 +1312,13 @@ __device__ void union_cell(const StreamDev &S, int A) {
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     const int total = a.cell_pre[a.n];
     const int nw = gridDim.x * kCellWaves;
+    WaveStats st;
     for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        union_cell(a.s[x.s], x.c);
+        stats_stream(a, st, x.s);
+        union_cell(a.s[x.s], x.c, st);
     }
+    stats_flush(a, st);
 }
 
 // Range count, point-parallel form (large batches, where the device is
@@ -1211,6 +1342,7 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
         c = S.pcell[p];
         dense = S.cflag[c] & kCellDense;
     }
+    int tests = 0;
     if (valid && !dense) {
         const float qx = S.sx[p];
         const float qy = dim > 1 ? S.sy[p] : 0.f;
@@ -1218,10 +1350,22 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
         walk_candidates(
             S, c,
             [&](int, float cx, float cy, float cz) {
+                tests++;
                 cnt += sqd(dim, qx, qy, qz, cx, cy, cz) < e2;
                 return cnt >= minpts;
             },
             ql.sub, LPQ);
+    }
+    if (a.stats) {
+        const unsigned long long t = (unsigned long long)wave_sum(tests);
+        const int nd = __popcll(__ballot(valid && dense && ql.sub == 0));
+        const int nq = __popcll(__ballot(valid && !dense && ql.sub == 0));
+        if (lane_id() == 0) {
+            unsigned long long *st = a.stats + (size_t)ql.x.s * kNumStats;
+            atomicAdd(st + kStatCountTests, t);
+            if (nd) atomicAdd(st + kStatDensePoints, (unsigned long long)nd);
+            if (nq) atomicAdd(st + kStatQueries, (unsigned long long)nq);
+        }
     }
     cnt = group_sum<LPQ>(cnt);
     if (valid && ql.sub == 0) {
@@ -1248,6 +1392,7 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const int dim = S.dim;
     const float e2 = S.eps2;
     int best = INT_MAX;
+    int tests = 0;
     if (mine) {
         const int c = S.pcell[p];
         const float qx = S.sx[p];
@@ -1256,11 +1401,16 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
         walk_candidates(
             S, c,
             [&](int j, float cx, float cy, float cz) {
+                tests++;
                 if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
                     best = min(best, S.canon[j]);
                 return false;
             },
             ql.sub, LPQ);
+    }
+    if (a.stats) {
+        const unsigned long long t = (unsigned long long)wave_sum(tests);
+        if (lane_id() == 0 && t) atomicAdd(a.stats + (size_t)ql.x.s * kNumStats + kStatBorderTests, t);
     }
     best = group_min<LPQ>(best);
     if (mine && ql.sub == 0) {
@@ -1741,6 +1891,7 @@ __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
     const int p = x.p0 + lane_id();
     if (g == a.pre_chunk[x.s] && lane_id() == 0) {
         StreamMut *m = S.mut;
+        if (a.stats) a.stats[(size_t)x.s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
         m->t = m->t + 1;
         m->u = m->u + 1 == S.units ? 0 : m->u + 1;
         m->epoch = m->epoch >= kEpochMax ? 1u : m->epoch + 1u;

@@ -35,6 +35,7 @@ struct Options {
     int check = -1;  // -1: as the reference for the format
     int cell_div = 0;
     bool phases = false;
+    bool stats = false;
 };
 
 [[noreturn]] void usage(const char *argv0) {
@@ -52,7 +53,8 @@ struct Options {
                  "         --check | --nc              Check the first warm-up pass against the brute-force\n"
                  "                                     validator, or not (default: tao, RBF and stock are checked)\n"
                  "         --cell-div <1|2|3>          Grid side Eps/k (default: automatic)\n"
-                 "         --phases                    Also print the time of every kernel phase\n",
+                 "         --phases                    Also print the time of every kernel phase\n"
+                 "         --stats                     Also print work counters per window (extra pass)\n",
                  argv0);
     std::exit(1);
 }
@@ -86,6 +88,7 @@ Options parse_args(int argc, char **argv) {
         else if (arg == "--nc") o.check = 0;
         else if (arg == "--cell-div") o.cell_div = std::atoi(value(i));
         else if (arg == "--phases") o.phases = true;
+        else if (arg == "--stats") o.stats = true;
         else if (arg == "--format") {
             o.format = parse_format(value(i));
             if (o.format < 0) usage(argv[0]);
@@ -226,6 +229,24 @@ int main(int argc, char **argv) {
         char *name = strtok_r(names, ",", &save);
         for (int i = 0; i < np && name; i++, name = strtok_r(nullptr, ",", &save))
             std::printf("[Time] %s: %g ms\n", name, n[i] ? ms[i] / (double)n[i] : 0.0);
+    }
+    if (o.stats && slides > 0) {
+        // the reference's DEBUG_INFO summary lines (print_overall_rt_info)
+        if ((rc = rtdbs_stream_set_timing(s, 0))) die("rtdbs_stream_set_timing", rc);
+        if ((rc = rtdbs_stream_set_stats(s, 1))) die("rtdbs_stream_set_stats", rc);
+        pass(false, false, nullptr);
+        rtdbs_stats st;
+        if ((rc = rtdbs_stream_get_stats(s, &st))) die("rtdbs_stream_get_stats", rc);
+        const double w = (double)st.slides;
+        std::printf("avg num spheres for each window: %lf\n", st.cells / w);
+        std::printf("avg num points in dense cells for each window: %lf\n", st.dense_points / w);
+        std::printf("avg num dist calculations for each window: %lf\n",
+                    (st.count_tests + st.merge_tests + st.border_tests) / w);
+        std::printf("avg counted points for each window: %lf\n", st.count_queries / w);
+        std::printf("avg dist calculations for each window: count %lf, merge %lf, border %lf\n", st.count_tests / w,
+                    st.merge_tests / w, st.border_tests / w);
+        std::printf("avg unions for each window: %lf (touching pairs %lf)%s\n", st.unites / w, st.merge_pairs / w,
+                    st.cell_parallel ? "" : " [not counted on the point-parallel path]");
     }
     if (check) std::printf("[Check] %lld windows equal to the brute-force validator\n", slides);
     rtdbs_stream_destroy(s);

@@ -51,7 +51,7 @@ def test_runner_checks_every_window(tmp_path, fmt, name, make, W, S, K, eps):
     path = tmp_path / name
     _write(path, fmt, make())
     r = subprocess.run([RUNNER, "-f", str(path), "-W", str(W), "-S", str(S), "-K", str(K), "-R", str(eps),
-                        "--warmup", "2", "--check", "--phases"], capture_output=True, text=True, timeout=300)
+                        "--warmup", "2", "--check", "--phases", "--stats"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     n = len(make())
     slides = (n - W) // S
@@ -59,3 +59,4 @@ def test_runner_checks_every_window(tmp_path, fmt, name, make, W, S, K, eps):
     m = re.search(r"\[Time\] total: ([0-9.eE+-]+) ms", r.stdout)
     assert m and float(m.group(1)) > 0
     assert "[Step] Warmup 1" in r.stdout and "[Time] count:" in r.stdout
+    assert "avg num dist calculations for each window:" in r.stdout
