@@ -118,7 +118,8 @@ struct LaunchShape {
     int32_t stiles;   // total stride tiles
     int32_t wtiles;   // total window tiles
     int32_t chunks;   // total chunks
-    int32_t persist_blocks;  // 256-thread blocks for the persistent row kernel
+    int32_t persist_blocks;  // 256-thread blocks for the persistent first-core kernel
+    int32_t rows_blocks;     // 256-thread blocks for the row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
     int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
     int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels

@@ -590,6 +590,10 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
+    // rows: a persistent grid of RTDBS_ROWS_PER_CU (default 8) blocks per CU;
+    // one thread per (cell, row) of the W*R bound measured slower
+    const int rcap = env_int("RTDBS_ROWS_PER_CU", 8, 1, 1 << 20);
+    b->shape.rows_blocks = (int)std::max(1ll, std::min((long long)device_cus() * rcap, pb));
     // Count and border: cell-parallel (coalesced candidate loads, more
     // instructions per query) while a launch is small and latency-bound,
     // point-parallel once its total window saturates the device.

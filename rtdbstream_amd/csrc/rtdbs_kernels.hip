@@ -229,15 +229,15 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Sorts the kTile keys of a 256-thread block ascending; thread t holds
-// positions 8t..8t+7 in v. Bitonic network: exchanges between a thread's own
-// elements stay in registers, exchanges within a wave use lane shuffles, and
-// only the three stages whose partners sit in another wave go through LDS
-// (`lds`, kTile keys) with block barriers.
-__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *lds) {
-    const int tid = threadIdx.x;
+// Sorts N keys ascending, 8 per thread: thread t (of N/8) holds positions
+// 8t..8t+7 in v. Bitonic network: exchanges between a thread's own elements
+// stay in registers, exchanges within a wave use lane shuffles, and only the
+// stages whose partners sit in another wave (N = 2048: three) go through LDS
+// (`lds`, N keys) with block barriers; N = 512 is one wave and needs none.
+template <int N>
+__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *lds, int tid) {
 #pragma unroll
-    for (int k = 2; k <= kTile; k <<= 1) {
+    for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             if (j >= 8) {
@@ -276,17 +276,18 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *ld
 }
 
 // Loads the packed keys (cell << slot_bits | slot) of points [tile0, tile0 +
-// kTile) of `in` (len points, slots from slot0) into v, 8 per thread, and
-// sorts them; padding keys are ~0 and sort last.
+// N) of `in` (len points, slots from slot0) into v, 8 per thread (tid < N/8),
+// and sorts them; padding keys are ~0 and sort last.
+template <int N>
 __device__ __forceinline__ void sort_tile_keys(const StreamDev &S, const float *in, int len, int tile0, int slot0,
-                                               uint64_t (&v)[8], uint64_t *lds) {
+                                               uint64_t (&v)[8], uint64_t *lds, int tid) {
     const int dim = S.dim;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const int gi = tile0 + threadIdx.x * 8 + e;
+        const int gi = tile0 + tid * 8 + e;
         v[e] = gi < len ? (cell_key_of(S, in + (size_t)gi * dim) << S.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
     }
-    bitonic_sort_regs(v, lds);
+    bitonic_sort_regs<N>(v, lds, tid);
 }
 
 // First window: keys and coordinates of each sorted tile (init ping-pong A).
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
     const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int tile0 = (g - a.pre_itile[s]) * kTile;
     uint64_t v[8];
-    sort_tile_keys(S, io.in, S.W, tile0, 0, v, lds);
+    sort_tile_keys<kTile>(S, io.in, S.W, tile0, 0, v, lds, threadIdx.x);
     const int nvalid = min(kTile, S.W - tile0);
     const uint32_t m = slot_mask(S);
     const int dim = S.dim;
@@ -343,10 +344,12 @@ __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds
     if (threadIdx.x == 0) S.tile_a[tile] = cnt;
 }
 
-// Launch 1 of a slide: blocks [0, stride tiles) sort the stride's tiles by
-// key (the reference's compute_cell_id + sortByCellIdAndOrder, aabb.cu:589-611,
-// thrust_helper.cu:21-29) into bkey0; the remaining blocks count the
-// survivors of each window tile.
+// Launch 1 of a slide: blocks [0, stride tiles) sort the stride by key (the
+// reference's compute_cell_id + sortByCellIdAndOrder, aabb.cu:589-611,
+// thrust_helper.cu:21-29) in runs of kRun keys, one run per wave, into bkey0;
+// the remaining blocks count the survivors of each window tile.
+constexpr int kRun = 512;  // keys per sorted run of the stride (one wave)
+
 __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
@@ -359,25 +362,15 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int tile0 = (g - a.pre_stile[s]) * kTile;
+    const int w = threadIdx.x >> 6;
     uint64_t v[8];
-    sort_tile_keys(S, io.in, S.S, tile0, S.mut->u * S.S, v, lds);
+    sort_tile_keys<kRun>(S, io.in, S.S, tile0 + w * kRun, S.mut->u * S.S, v, nullptr, lane_id());
     // coalesced store through LDS
-    __syncthreads();
 #pragma unroll
     for (int e = 0; e < 8; e++) lds[threadIdx.x * 8 + e] = v[e];
     __syncthreads();
     const int nvalid = min(kTile, S.S - tile0);
     for (int i = threadIdx.x; i < nvalid; i += kTileThreads) S.bkey0[tile0 + i] = lds[i];
-}
-
-// Number of keys below `key` among lds[0, n), n <= kTile (branch-free
-// binary search over a sorted LDS array).
-__device__ __forceinline__ int lds_lower_bound(const uint64_t *lds, int n, uint64_t key) {
-    int pos = 0;
-#pragma unroll
-    for (int step = kTile; step > 0; step >>= 1)
-        if (pos + step <= n && lds[pos + step - 1] < key) pos += step;
-    return pos;
 }
 
 // Ordered compaction of the survivors into the A arrays. Each tile's offset
@@ -422,105 +415,82 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
     }
 }
 
-// Launch 2 of a slide: blocks [0, stride tiles) place every key of their
-// sorted tile at its rank in the whole stride (its index in the tile plus,
-// for each other tile, the number of that tile's keys below it) and write
-// keys and coordinates to the B1 arrays - the stride sorted by (cell, slot);
-// the remaining blocks compact the window's survivors. The other tiles are
-// staged kRankGroup at a time, so a stride of up to kRankGroup + 1 tiles
-// costs one load round trip, and the binary searches of all staged tiles run
-// interleaved.
-constexpr int kRankGroup = 4;
+// Launch 2 of a slide: kRankParts blocks per stride tile place each key at
+// its rank in the whole stride - its index in its sorted run plus, for every
+// other run, the number of that run's keys below it - and write keys and
+// coordinates to the B1 arrays: the stride sorted by (cell, slot). The other
+// runs are staged in LDS kStageRuns at a time and the binary searches of all
+// staged runs advance together; one key per thread.
+constexpr int kRankParts = kTile / kTileThreads;
+constexpr int kStageRuns = 8;  // 32 KB of LDS
 
-__global__ __launch_bounds__(kTileThreads) void k_rank_compact(BatchArgs a) {
-    __shared__ uint64_t lds[kRankGroup * kTile];
+__global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
+    __shared__ uint64_t lds[kStageRuns * kRun];
     const int g = blockIdx.x;
-    const int nst = a.pre_stile[a.n];
-    if (g >= nst) {
-        expire_compact(a, g - nst, (int *)lds);
-        return;
-    }
-    const int s = find_stream(a.pre_stile, a.n, g);
+    const int gb = g / kRankParts, part = g % kRankParts;
+    const int s = find_stream(a.pre_stile, a.n, gb);
     const StreamDev &S = a.s[s];
-    const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int t0 = a.pre_stile[s], nt = a.pre_stile[s + 1] - t0;
-    const int tile = g - t0, tile0 = tile * kTile;
-    const int nvalid = min(kTile, S.S - tile0);
-    uint64_t key[8];
-    int rank[8];
+    const int i = (gb - a.pre_stile[s]) * kTile + part * kTileThreads + threadIdx.x;
+    const bool valid = i < S.S;
+    const uint64_t key = valid ? S.bkey0[i] : ~0ull;
+    const int myrun = valid ? i / kRun : -1, nr = (S.S + kRun - 1) / kRun;
+    int rank = valid ? i - myrun * kRun : 0;
+    // the runs other than this block's own ones are staged; a block's keys
+    // share at most one run (kTileThreads divides kRun)
+    const int blockrun = ((gb - a.pre_stile[s]) * kTile + part * kTileThreads) / kRun;
+    for (int o0 = 0; o0 < nr - 1; o0 += kStageRuns) {
+        int len[kStageRuns];
+        constexpr int per = kStageRuns * kRun / kTileThreads;
+        uint64_t stage[per];  // every load issued before the LDS stores
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int i = threadIdx.x + e * kTileThreads;
-        key[e] = i < nvalid ? S.bkey0[tile0 + i] : ~0ull;
-        rank[e] = i;
-    }
-    // other tiles of the stream, kRankGroup per round
-    for (int o0 = 0; o0 < nt - 1; o0 += kRankGroup) {
-        int n2[kRankGroup];
-        uint64_t stage[kRankGroup][8];  // every load issued before any LDS store
-#pragma unroll
-        for (int q = 0; q < kRankGroup; q++) {
+        for (int q = 0; q < kStageRuns; q++) {
             const int o = o0 + q;
-            const int t = o < tile ? o : o + 1;  // the o-th tile other than this one
-            n2[q] = o < nt - 1 ? min(kTile, S.S - t * kTile) : 0;
+            const int r = o < blockrun ? o : o + 1;  // the o-th run other than the block's
+            len[q] = o < nr - 1 ? min(kRun, S.S - r * kRun) : 0;
+        }
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int i = threadIdx.x + e * kTileThreads;
-                stage[q][e] = i < n2[q] ? S.bkey0[t * kTile + i] : ~0ull;
+        for (int e = 0; e < per; e++) {
+            const int slot = threadIdx.x + e * kTileThreads;  // [0, kStageRuns * kRun)
+            const int q = slot / kRun, j = slot % kRun;
+            const int o = o0 + q;
+            const int r = o < blockrun ? o : o + 1;
+            stage[e] = (o < nr - 1 && j < min(kRun, S.S - r * kRun)) ? S.bkey0[r * kRun + j] : ~0ull;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < per; e++) lds[threadIdx.x + e * kTileThreads] = stage[e];
+        __syncthreads();
+        int pos[kStageRuns];
+#pragma unroll
+        for (int q = 0; q < kStageRuns; q++) pos[q] = 0;
+#pragma unroll
+        for (int step = kRun; step > 0; step >>= 1) {
+#pragma unroll
+            for (int q = 0; q < kStageRuns; q++) {
+                const int probe = pos[q] + step;
+                const uint64_t v = lds[q * kRun + min(probe, kRun) - 1];
+                if (probe <= len[q] && v < key) pos[q] = probe;
             }
         }
-        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kRankGroup; q++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) lds[q * kTile + threadIdx.x + e * kTileThreads] = stage[q][e];
-        __syncthreads();
-        // branch-free binary searches of all (tile, key) pairs, advanced one
-        // step together so their LDS reads overlap
-        int pos[kRankGroup][8];
-#pragma unroll
-        for (int q = 0; q < kRankGroup; q++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) pos[q][e] = 0;
-#pragma unroll
-        for (int step = kTile; step > 0; step >>= 1) {
-#pragma unroll
-            for (int q = 0; q < kRankGroup; q++)
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int probe = pos[q][e] + step;
-                    const uint64_t v = lds[q * kTile + min(probe, kTile) - 1];
-                    if (probe <= n2[q] && v < key[e]) pos[q][e] = probe;
-                }
-        }
-#pragma unroll
-        for (int q = 0; q < kRankGroup; q++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) rank[e] += pos[q][e];
+        for (int q = 0; q < kStageRuns; q++) rank += pos[q];
     }
-    const uint32_t m = slot_mask(S);
-    const int slot0 = S.mut->u * S.S, dim = S.dim;
-    float px[8], py[8], pz[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int i = threadIdx.x + e * kTileThreads;
-        const int gi = i < nvalid ? (int)((uint32_t)key[e] & m) - slot0 : 0;
-        const float *p = io.in + (size_t)gi * dim;
-        px[e] = p[0];
-        py[e] = dim > 1 ? p[1] : 0.f;
-        pz[e] = dim > 2 ? p[2] : 0.f;
-    }
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int i = threadIdx.x + e * kTileThreads;
-        if (i < nvalid) {
-            const int r = rank[e];
-            S.bkey1[r] = key[e];
-            S.bx1[r] = px[e];
-            if (dim > 1) S.by1[r] = py[e];
-            if (dim > 2) S.bz1[r] = pz[e];
-        }
-    }
+    if (!valid) return;
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    const int dim = S.dim;
+    const int gi = (int)((uint32_t)key & slot_mask(S)) - S.mut->u * S.S;
+    const float *p = io.in + (size_t)gi * dim;
+    const float px = p[0], py = dim > 1 ? p[1] : 0.f, pz = dim > 2 ? p[2] : 0.f;
+    S.bkey1[rank] = key;
+    S.bx1[rank] = px;
+    if (dim > 1) S.by1[rank] = py;
+    if (dim > 2) S.bz1[rank] = pz;
+}
+
+// Launch 3: ordered compaction of the window's survivors.
+__global__ __launch_bounds__(kTileThreads) void k_compact(BatchArgs a) {
+    __shared__ int lds4[4];
+    expire_compact(a, blockIdx.x, lds4);
 }
 
 // Number of elements of A among the first d of merge(A, B); keys are unique.
@@ -1965,7 +1935,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_sort_expire, sh.stiles + sh.wtiles, kTileThreads, a);
     PH_END(kPhStrideSort);
     PH_BEGIN(kPhMerge);
-    RTDBS_LAUNCH(k_rank_compact, sh.stiles + sh.wtiles, kTileThreads, a);
+    RTDBS_LAUNCH(k_rank, sh.stiles * kRankParts, kTileThreads, a);
+    RTDBS_LAUNCH(k_compact, sh.wtiles, kTileThreads, a);
     PH_END(kPhMerge);
     PH_BEGIN(kPhMergeFinal);
     RTDBS_LAUNCH(k_merge_final, sh.wtiles, kTileThreads, a);
@@ -1974,7 +1945,7 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_cells, sh.wtiles, kTileThreads, a);
     PH_END(kPhCells);
     PH_BEGIN(kPhRows);
-    RTDBS_LAUNCH(k_rows, sh.persist_blocks, kTileThreads, a);
+    RTDBS_LAUNCH(k_rows, sh.rows_blocks, kTileThreads, a);
     PH_END(kPhRows);
     {
         // The range-count kernel is timed by its own dispatch: the events take
