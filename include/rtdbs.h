@@ -271,8 +271,9 @@ typedef struct {
     int64_t border_tests;  /* distance tests of the border pass */
     int64_t unites;        /* union-find unite calls */
     int64_t merge_pairs;   /* touching element pairs found by the merge */
-    int32_t cell_parallel; /* 1: counted on the cell-parallel kernels; 0: point-parallel, where
-                              merge_tests, unites and merge_pairs are not counted */
+    int32_t cell_parallel; /* 1: count and border ran cell-parallel; 0: point-parallel (large
+                              batches). The merge counters come from the cell-parallel merge,
+                              which both paths use by default */
     int32_t pad;
     int64_t reserved[6];
 } rtdbs_stats;

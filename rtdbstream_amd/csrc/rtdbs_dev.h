@@ -124,6 +124,7 @@ struct LaunchShape {
     int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
     int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels
     int32_t cellpar;         // 1: cell-parallel count/border, 0: point-parallel
+    int32_t union_cells;     // point-parallel path: 1 cell-level merge kernels, 0 the cell-parallel merge
 };
 
 enum Phase {

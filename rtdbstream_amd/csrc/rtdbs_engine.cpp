@@ -590,9 +590,9 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
-    // rows: a persistent grid of RTDBS_ROWS_PER_CU (default 8) blocks per CU;
-    // one thread per (cell, row) of the W*R bound measured slower
-    const int rcap = env_int("RTDBS_ROWS_PER_CU", 8, 1, 1 << 20);
+    // rows: a persistent grid of RTDBS_ROWS_PER_CU blocks per CU (measured
+    // best: 8 for a latency-bound launch, 32 for a large batch)
+    const int rcap = env_int("RTDBS_ROWS_PER_CU", cells_total <= kCellParMaxPoints ? 8 : 32, 1, 1 << 20);
     b->shape.rows_blocks = (int)std::max(1ll, std::min((long long)device_cus() * rcap, pb));
     // Count and border: cell-parallel (coalesced candidate loads, more
     // instructions per query) while a launch is small and latency-bound,
@@ -600,6 +600,9 @@ int refresh_batch(rtdbs_batch *b) {
     // RTDBS_NEIGHBOR_MODE: 1 forces cell-parallel, 2 point-parallel.
     const int nmode = env_int("RTDBS_NEIGHBOR_MODE", 0, 0, 2);
     b->shape.cellpar = nmode == 1 || (nmode == 0 && cells_total <= kCellParMaxPoints);
+    // the cell-parallel merge also wins in large batches (RTDBS_UNION_CELLS=1
+    // selects the older cell-level merge kernels there)
+    b->shape.union_cells = env_int("RTDBS_UNION_CELLS", 0, 0, 1);
     b->shape.cellpar_blocks = (int)std::max(
         1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", 16, 1, 1024), (cells_total + 3) / 4));
     b->shape.cell_blocks = (int)std::max(

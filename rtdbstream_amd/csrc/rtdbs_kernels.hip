@@ -1255,6 +1255,17 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
             const unsigned chunk = (unsigned)min(64, TC - jb);
             uint64_t m = qm;
             if (safeA) {
+                // lanes holding one element form a run (a safe cell's points are
+                // adjacent); one touching lane settles its whole run
+                const int lane = lane_id();
+                const int prev_rep = __shfl_up(rep, 1);  // every lane takes part in the shuffles
+                const int prev_core = __shfl_up((int)bcore, 1);
+                const uint64_t starts = __ballot(lane == 0 || prev_rep != rep || !prev_core);
+                const uint64_t after = starts & ~((2ull << lane) - 1ull);  // run starts above this lane
+                const uint64_t upto = after ? (after & (~after + 1ull)) - 1ull : ~0ull;
+                const uint64_t below = starts & ((2ull << lane) - 1ull);
+                const uint64_t from = ~((1ull << (63 - __clzll((long long)below))) - 1ull);
+                const uint64_t run = upto & from;  // this lane's run
                 bool touched = false;
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
@@ -1262,7 +1273,8 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
                     st.v[kStatMergeTests] += chunk;
                     touched = touched ||
                               (bcore && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
-                    if (__ballot(touched) == bm) break;  // every candidate core already touches A
+                    const uint64_t t = __ballot(touched);
+                    if (__ballot(!bcore || (t & run) != 0ull) == ~0ull) break;  // every candidate run settled
                 }
                 unite_once_per_rep(S.parent, touched, rep, repA, st);
             } else {
@@ -1416,6 +1428,26 @@ __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
                 }
         }
         S.fcore[c] = fc;
+    }
+}
+
+// Union-find element of every position for the cell-parallel merge when the
+// point-parallel count ran (it does not know the cells' first cores): the
+// representative's slot in a safe cell, else the position's own slot.
+__global__ __launch_bounds__(kTileThreads) void k_rep(BatchArgs a) {
+    const int g = blockIdx.x;
+    const int s = find_stream(a.pre_wtile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const int p0 = (g - a.pre_wtile[s]) * kWTile;
+#pragma unroll
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
+        const int p = p0 + r * kTileThreads + threadIdx.x;
+        if (p < S.W) {
+            const int c = S.pcell[p];
+            const bool safe = S.cflag[c] & kCellSafe;
+            const int fc = S.fcore[c];
+            S.rep[p] = slot_of(S, safe && fc != kNoCore ? fc : p);
+        }
     }
 }
 
@@ -1962,10 +1994,15 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     PH_BEGIN(kPhUnion);
     if (sh.cellpar) {
         RTDBS_LAUNCH(k_union, sh.cellpar_blocks, 64 * kCellWaves, a);
-    } else {
+    } else if (sh.union_cells) {
         RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
         RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
         RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
+    } else {
+        // point-parallel count, cell-parallel merge
+        RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
+        RTDBS_LAUNCH(k_rep, sh.wtiles, kTileThreads, a);
+        RTDBS_LAUNCH(k_union, sh.cellpar_blocks, 64 * kCellWaves, a);
     }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);

@@ -245,8 +245,7 @@ int main(int argc, char **argv) {
         std::printf("avg counted points for each window: %lf\n", st.count_queries / w);
         std::printf("avg dist calculations for each window: count %lf, merge %lf, border %lf\n", st.count_tests / w,
                     st.merge_tests / w, st.border_tests / w);
-        std::printf("avg unions for each window: %lf (touching pairs %lf)%s\n", st.unites / w, st.merge_pairs / w,
-                    st.cell_parallel ? "" : " [not counted on the point-parallel path]");
+        std::printf("avg unions for each window: %lf (touching pairs %lf)\n", st.unites / w, st.merge_pairs / w);
     }
     if (check) std::printf("[Check] %lld windows equal to the brute-force validator\n", slides);
     rtdbs_stream_destroy(s);

@@ -191,12 +191,15 @@ def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
     s.close()
 
 
-@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("mode", ["1", "2", "2u"])
 @pytest.mark.parametrize("dim,W,S,eps,mp,k,seed", [SMALL[2], SMALL[6], SMALL[9], SMALL[13]])
 def test_neighbour_kernel_forms_agree(engine, oracle, monkeypatch, mode, dim, W, S, eps, mp, k, seed):
-    """Cell-parallel (1) and point-parallel (2) count/border kernels both
-    reproduce the oracle (launches pick one by size; both are forced here)."""
-    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    """Cell-parallel (1) and point-parallel (2) count/border kernels, and the
+    cell-level merge kernels (2u), all reproduce the oracle (launches pick a
+    form by size; each is forced here)."""
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode[0])
+    if mode == "2u":
+        monkeypatch.setenv("RTDBS_UNION_CELLS", "1")
     slides = 4
     data = _mixed(W + slides * S, dim, seed)
     ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")

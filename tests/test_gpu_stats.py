@@ -41,10 +41,8 @@ def test_counter_invariants(engine, monkeypatch, mode):
     assert st["count_tests"] >= st["count_queries"]
     assert st["border_tests"] > 0
     assert st["cell_parallel"] == (1 if mode == "1" else 0)
-    if mode == "1":
-        assert st["merge_tests"] > 0 and 0 < st["unites"] <= st["merge_pairs"]
-    else:
-        assert st["merge_tests"] == 0 and st["unites"] == 0
+    # both paths merge with the cell-parallel kernel
+    assert st["merge_tests"] > 0 and 0 < st["unites"] <= st["merge_pairs"]
 
 
 def test_batch_counters_per_member(engine):
