@@ -1176,10 +1176,27 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
     stats_flush(a, st);
 }
 
-// Unites `target` with `rep` for every lane that wants it, once per run of
-// lanes holding the same representative (a safe cell's points are adjacent
-// in a candidate chunk and share one representative).
-__device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, int rep, int target, WaveStats &st) {
+// Touching (element, element) pairs of one wave, united in parallel rounds:
+// the pairs of several candidate chunks and cells are collected in LDS and
+// each lane then walks the union-find for one pair, so the dependent loads of
+// the walks overlap instead of adding up chunk after chunk.
+constexpr int kEdgeCap = 256;
+
+struct EdgeBuf {
+    int2 *e;      // this wave's LDS slice
+    int n;        // pairs held (uniform)
+    int32_t *parent;  // union-find of the stream the pairs belong to
+};
+
+__device__ __forceinline__ void edge_flush(EdgeBuf &eb) {
+    for (int i = lane_id(); i < eb.n; i += 64) uf_unite(eb.parent, eb.e[i].x, eb.e[i].y);
+    eb.n = 0;
+}
+
+// Queues (target, rep) for every lane that wants it, once per run of lanes
+// holding the same representative (a safe cell's points are adjacent in a
+// candidate chunk and share one representative).
+__device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int rep, int target, WaveStats &st) {
     const int lane = lane_id();
     const uint64_t m = __ballot(want);
     if (m == 0ull) return;
@@ -1187,9 +1204,13 @@ __device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, i
     const int prev = below ? 63 - __clzll((long long)below) : lane;
     const int prep = __shfl(rep, prev);
     const bool first = want && (below == 0ull || prep != rep);
+    const uint64_t fm = __ballot(first);
+    const int nf = __popcll(fm);
     st.v[kStatMergePairs] += __popcll(m);
-    st.v[kStatUnites] += __popcll(__ballot(first));
-    if (first) uf_unite(parent, target, rep);
+    st.v[kStatUnites] += nf;
+    if (eb.n + nf > kEdgeCap) edge_flush(eb);
+    if (first) eb.e[eb.n + __popcll(fm & ((1ull << lane) - 1ull))] = make_int2(target, rep);
+    eb.n += nf;
 }
 
 // Cluster merge (the reference's __raygen__cluster + unite,
@@ -1202,7 +1223,7 @@ __device__ __forceinline__ void unite_once_per_rep(int32_t *parent, bool want, i
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
-__device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
+__device__ void union_cell(const StreamDev &S, int A, WaveStats &st, EdgeBuf &eb) {
     const int lane = lane_id();
     const int fcA = S.fcore[A];
     if (fcA == kNoCore) return;
@@ -1276,7 +1297,7 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
                     const uint64_t t = __ballot(touched);
                     if (__ballot(!bcore || (t & run) != 0ull) == ~0ull) break;  // every candidate run settled
                 }
-                unite_once_per_rep(S.parent, touched, rep, repA, st);
+                unite_once_per_rep(eb, touched, rep, repA, st);
             } else {
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
@@ -1284,7 +1305,7 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
                     st.v[kStatMergeTests] += chunk;
                     const bool hit = bcore && pos < cs + q0 + qi &&
                                      sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
-                    unite_once_per_rep(S.parent, hit, rep, lane_i(qslot, qi), st);
+                    unite_once_per_rep(eb, hit, rep, lane_i(qslot, qi), st);
                 }
             }
         }
@@ -1292,14 +1313,23 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st) {
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
+    __shared__ int2 edges[kCellWaves][kEdgeCap];
     const int total = a.cell_pre[a.n];
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
+    EdgeBuf eb{edges[threadIdx.x >> 6], 0, nullptr};
+    int es = -1;  // stream of the queued pairs
     for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
         stats_stream(a, st, x.s);
-        union_cell(a.s[x.s], x.c, st);
+        if (x.s != es) {
+            if (es >= 0) edge_flush(eb);
+            es = x.s;
+            eb.parent = a.s[x.s].parent;
+        }
+        union_cell(a.s[x.s], x.c, st, eb);
     }
+    if (es >= 0) edge_flush(eb);
     stats_flush(a, st);
 }
 
