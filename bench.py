@@ -47,7 +47,20 @@ def window_at(data, W, S, t):
     return win
 
 
-def plain_leg(R, WL, name, steps, warmup):
+def tune(R, dim, W, S, eps, mp, data, enabled):
+    """Grid side and kernel form from the project's autotuner (trial slides
+    on the stream's first W + 9S points); the defaults when disabled."""
+    if not enabled:
+        return {"cell_div": 0, "lanes_per_query": 0, "neighbor_mode": 0, "from_cache": False, "trials": 0}
+    return R.autotune(dim, W, S, eps, mp, data[:W + 9 * S], slides=8)
+
+
+def tuned_stream(R, dim, W, S, eps, mp, t):
+    return R.Stream(dim, W, S, eps, mp, cell_div=t["cell_div"], lanes_per_query=t["lanes_per_query"],
+                    neighbor_mode=t["neighbor_mode"])
+
+
+def plain_leg(R, WL, name, steps, warmup, autotune=True):
     cfg = WL.WORKLOADS[name]
     W, S, dim, eps, mp = cfg["window"], cfg["stride"], cfg["dim"], cfg["eps"], cfg["min_pts"]
     total_slides = warmup + steps
@@ -57,7 +70,10 @@ def plain_leg(R, WL, name, steps, warmup):
     d_data.upload(data)
     d_label = R.DeviceBuffer(W * 4)
     d_cid = R.DeviceBuffer(W * 4)
-    s = R.Stream(dim, W, S, eps, mp)
+    t0 = time.perf_counter()
+    tuned = tune(R, dim, W, S, eps, mp, data, autotune)
+    tune_s = time.perf_counter() - t0
+    s = tuned_stream(R, dim, W, S, eps, mp, tuned)
     s.init(data[:W])
 
     def slide(t):
@@ -86,7 +102,9 @@ def plain_leg(R, WL, name, steps, warmup):
     phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in s.timing().items()}
     s.close()
     return dict(data=data, cfg=cfg, cfg_name=name, elapsed=elapsed, lat=lat, timing=timing, info=info, W=W, S=S, dim=dim,
-                phase_ms=phase_ms,
+                phase_ms=phase_ms, tuned={k: tuned[k] for k in ("cell_div", "lanes_per_query", "neighbor_mode",
+                                                                 "from_cache", "trials")},
+                tune_s=round(tune_s, 2),
                 eps=eps, min_pts=mp, warmup=warmup, steps=steps)
 
 
@@ -183,7 +201,7 @@ def cpu_baseline(R, WL, res, seconds, max_slides=400):
             "slides_per_s": round(n_done / cpu_time, 3)}, exact, n_done
 
 
-def batched_leg(R, WL, n_streams, slides, warmup):
+def batched_leg(R, WL, n_streams, slides, warmup, autotune=True):
     specs = WL.multi_stream_specs(n_streams=n_streams, slides=slides + warmup + 2)
     streams, bufs, datas = [], [], []
     for sp in specs:
@@ -193,9 +211,14 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         db.upload(d)
         bufs.append((db, R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4),
                      R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4)))
+    # every stream's grid side and kernel form from the autotuner (both legs)
+    t0 = time.perf_counter()
+    tuned = [tune(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], d, autotune)
+             for sp, d in zip(specs, datas)]
+    tune_s = time.perf_counter() - t0
     # plain: one stream at a time
-    for sp, d in zip(specs, datas):
-        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+    for sp, d, tu in zip(specs, datas, tuned):
+        s = tuned_stream(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], tu)
         s.init(d[:sp["window"]])
         streams.append(s)
 
@@ -222,8 +245,8 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         s.close()
     # batched: same streams from the same first windows, one graph replay per slide
     streams = []
-    for sp, d in zip(specs, datas):
-        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+    for sp, d, tu in zip(specs, datas, tuned):
+        s = tuned_stream(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], tu)
         s.init(d[:sp["window"]])
         streams.append(s)
     b = R.Batch(streams, use_graph=True)
@@ -273,8 +296,14 @@ def batched_leg(R, WL, n_streams, slides, warmup):
         s.close()
     pts = sum(sp["stride"] for sp in specs) * slides
     nsl = len(specs) * slides
+    forms = {}
+    for tu in tuned:
+        f = "k%d/lpq%d/mode%d" % (tu["cell_div"], tu["lanes_per_query"], tu["neighbor_mode"])
+        forms[f] = forms.get(f, 0) + 1
     return {
         "streams": len(specs), "slides_per_stream": slides,
+        "autotune": {"forms": forms, "from_cache": sum(bool(tu["from_cache"]) for tu in tuned),
+                     "tune_s": round(tune_s, 2)},
         "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
         "plain_points_per_s": round(pts / t_plain, 1), "plain_slides_per_s": round(nsl / t_plain, 1),
         "batched_points_per_s": round(pts / t_batch, 1), "batched_slides_per_s": round(nsl / t_batch, 1),
@@ -381,6 +410,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this size (profiler byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default grid side and kernel form instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs every stream on a single MI355X "
@@ -401,7 +432,7 @@ def main():
         log("calibration copy: %d MiB read + %d MiB written by k_copy16" % (args.calib_copy_mb, args.calib_copy_mb))
         del src, dst
     log("plain leg: %s, %d warmup + %d timed slides" % (args.workload, args.warmup, args.steps))
-    res = plain_leg(R, WL, args.workload, args.steps, args.warmup)
+    res = plain_leg(R, WL, args.workload, args.steps, args.warmup, autotune=not args.no_autotune)
     W, S = res["W"], res["S"]
     ms_per = res["elapsed"] / args.steps * 1e3
     out = {
@@ -421,6 +452,7 @@ def main():
         "p99_ms": round(float(np.percentile(res["lat"], 99)), 4),
         "phase_ms": res["phase_ms"],
         "grid": {"cell_div": res["info"]["cell_div"], "cells_last": res["info"]["cells"]},
+        "autotune": dict(res["tuned"], tune_s=res["tune_s"]),
     }
     log("plain: %.1f points/s, %.3f ms/slide" % (out["value"], ms_per))
     out["roofline"] = roofline(WL, res)
@@ -432,7 +464,8 @@ def main():
         log("cpu baseline: %s exact=%s" % (cb, exact))
     if not args.no_batch:
         log("batched leg: %d streams x %d slides" % (args.batch_streams, args.batch_slides))
-        out["batched"] = batched_leg(R, WL, args.batch_streams, args.batch_slides, 2)
+        out["batched"] = batched_leg(R, WL, args.batch_streams, args.batch_slides, 2,
+                                     autotune=not args.no_autotune)
         out["roofline_batched"] = batched_roofline(WL, out["batched"])
         log("batched: %s" % out["batched"])
         log("roofline (batched launch): %s" % out["roofline_batched"])

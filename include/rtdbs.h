@@ -60,8 +60,10 @@ typedef struct rtdbs_params {
     float lo[3];      /* grid bounds (reference: whole-file min/max, util.cpp:23-55);   */
     float hi[3];      /* if hi[a] <= lo[a] on a used axis they come from the first window */
     int32_t cell_div; /* stencil reach k (grid side = Eps/k), 1..3; 0 = default (2) */
-    int32_t lanes_per_query; /* threads per query point in the plain path: 1, 2, 4; 0 = auto */
-    int32_t reserved[6];
+    int32_t lanes_per_query; /* threads per query of the point-parallel kernels: 1, 2, 4; 0 = auto */
+    int32_t neighbor_mode;   /* count/border kernels of a single-stream launch: 1 cell-parallel,
+                                2 point-parallel; 0 = by launch size */
+    int32_t reserved[5];
 } rtdbs_params;
 
 typedef struct rtdbs_stream rtdbs_stream;
@@ -136,8 +138,9 @@ void rtdbs_batch_destroy(rtdbs_batch *b);
 /* ---- autotuner ------------------------------------------------------------ */
 /* No reference counterpart (the reference fixes the grid side to Eps/sqrt(3),
  * dbscan.cpp:34). Times trial slides of the given stream sample for every
- * stencil reach k (grid side Eps/k) and lanes-per-query setting and returns
- * the fastest. `points` holds W + slides*S points (slides >= 1). Results are
+ * stencil reach k (grid side Eps/k), then for each form of the neighbourhood
+ * kernels (cell-parallel; point-parallel with 1, 2 or 4 lanes per query) at
+ * the best reach, and returns the fastest. `points` holds W + slides*S points (slides >= 1). Results are
  * cached by (dim, W, S, MinPts, density) in the file named by the
  * RTDBS_TUNE_CACHE environment variable (default
  * $HOME/.cache/rtdbs_autotune.txt); a cache hit runs no trials. */
@@ -148,6 +151,8 @@ typedef struct rtdbs_tune_result {
     int32_t from_cache;
     int32_t trials;
     char key[96];
+    int32_t neighbor_mode; /* 1 cell-parallel, 2 point-parallel (with lanes_per_query) */
+    int32_t pad;
 } rtdbs_tune_result;
 
 int rtdbs_autotune(const rtdbs_params *params, const float *points, int64_t n_points, int32_t slides,

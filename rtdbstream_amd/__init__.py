@@ -43,7 +43,8 @@ class Params(ctypes.Structure):
         ("dim", ctypes.c_int32), ("window", ctypes.c_int32), ("stride", ctypes.c_int32),
         ("eps", ctypes.c_float), ("min_pts", ctypes.c_int32),
         ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
-        ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("neighbor_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -51,6 +52,7 @@ class TuneResult(ctypes.Structure):
     _fields_ = [
         ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("ms_per_slide", ctypes.c_double),
         ("from_cache", ctypes.c_int32), ("trials", ctypes.c_int32), ("key", ctypes.c_char * 96),
+        ("neighbor_mode", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
 
@@ -191,7 +193,8 @@ def device_count():
     return n.value
 
 
-def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0):
+def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0,
+                neighbor_mode=0):
     p = Params()
     p.dim, p.window, p.stride, p.eps, p.min_pts = int(dim), int(window), int(stride), float(eps), int(min_pts)
     for a in range(3):
@@ -199,6 +202,7 @@ def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0,
         p.hi[a] = float(hi[a]) if hi is not None and a < len(hi) else 0.0
     p.cell_div = int(cell_div)
     p.lanes_per_query = int(lanes_per_query)
+    p.neighbor_mode = int(neighbor_mode)
     return p
 
 
@@ -209,8 +213,9 @@ def autotune(dim, window, stride, eps, min_pts, points, slides=3, lo=None, hi=No
     pts = np.ascontiguousarray(points, dtype=np.float32)
     r = TuneResult()
     _check(lib().rtdbs_autotune(ctypes.byref(p), _ptr(pts), pts.size // int(dim), int(slides), ctypes.byref(r)))
-    return {"cell_div": r.cell_div, "lanes_per_query": r.lanes_per_query, "ms_per_slide": r.ms_per_slide,
-            "from_cache": bool(r.from_cache), "trials": r.trials, "key": r.key.decode()}
+    return {"cell_div": r.cell_div, "lanes_per_query": r.lanes_per_query, "neighbor_mode": r.neighbor_mode,
+            "ms_per_slide": r.ms_per_slide, "from_cache": bool(r.from_cache), "trials": r.trials,
+            "key": r.key.decode()}
 
 
 def bruteforce(points, eps, min_pts, dim=None):
@@ -277,8 +282,10 @@ class DeviceBuffer:
 class Stream:
     """One sliding window (the reference's ScanState + slide loop body)."""
 
-    def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0):
-        self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div, lanes_per_query)
+    def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0,
+                 neighbor_mode=0):
+        self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div, lanes_per_query,
+                                  neighbor_mode)
         self.dim, self.W, self.S = int(dim), int(window), int(stride)
         h = ctypes.c_void_p()
         _check(lib().rtdbs_stream_create(ctypes.byref(self.params), ctypes.byref(h)))

@@ -104,6 +104,8 @@ int validate_params(const rtdbs_params *p) {
         return fail(RTDBS_EINVAL, "cell_div must be 0 (auto) or 1..%d (got %d)", kMaxK, p->cell_div);
     if (!(p->lanes_per_query == 0 || p->lanes_per_query == 1 || p->lanes_per_query == 2 || p->lanes_per_query == 4))
         return fail(RTDBS_EINVAL, "lanes_per_query must be 0 (auto), 1, 2 or 4 (got %d)", p->lanes_per_query);
+    if (p->neighbor_mode < 0 || p->neighbor_mode > 2)
+        return fail(RTDBS_EINVAL, "neighbor_mode must be 0 (auto), 1 or 2 (got %d)", p->neighbor_mode);
     for (int a = 0; a < p->dim; a++)
         if (!std::isfinite(p->lo[a]) || !std::isfinite(p->hi[a]))
             return fail(RTDBS_EINVAL, "grid bounds must be finite");
@@ -597,9 +599,12 @@ int refresh_batch(rtdbs_batch *b) {
     // Count and border: cell-parallel (coalesced candidate loads, more
     // instructions per query) while a launch is small and latency-bound,
     // point-parallel once its total window saturates the device.
-    // RTDBS_NEIGHBOR_MODE: 1 forces cell-parallel, 2 point-parallel.
+    // RTDBS_NEIGHBOR_MODE: 1 forces cell-parallel, 2 point-parallel (over a
+    // single stream's own neighbor_mode).
     const int nmode = env_int("RTDBS_NEIGHBOR_MODE", 0, 0, 2);
-    b->shape.cellpar = nmode == 1 || (nmode == 0 && cells_total <= kCellParMaxPoints);
+    const int pmode = n == 1 ? b->streams[0]->p.neighbor_mode : 0;  // a single stream's own setting
+    const int mode = nmode ? nmode : pmode;
+    b->shape.cellpar = mode == 1 || (mode == 0 && cells_total <= kCellParMaxPoints);
     // the cell-parallel merge also wins in large batches (RTDBS_UNION_CELLS=1
     // selects the older cell-level merge kernels there)
     b->shape.union_cells = env_int("RTDBS_UNION_CELLS", 0, 0, 1);

@@ -1,6 +1,7 @@
 // Autotuner: picks the grid side (stencil reach k, side = Eps/k) and the
-// lanes-per-query setting of the point-parallel kernels by timing trial slides
-// on a sample of the stream, and remembers the choice by a workload signature.
+// form of the neighbourhood kernels (cell-parallel, or point-parallel with 1,
+// 2 or 4 lanes per query) by timing trial slides on a sample of the stream,
+// and remembers the choice by a workload signature.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,18 +63,22 @@ std::string make_key(const rtdbs_params &p, const float *pts) {
     return buf;
 }
 
-bool cache_lookup(const std::string &key, int *k, int *lpq, double *ms) {
+struct Choice {
+    int k = 2, lpq = 0, mode = 0;
+    double ms = 0.0;
+};
+
+// Cache lines: "key k lanes_per_query neighbor_mode ms"; the last entry of a
+// key wins.
+bool cache_lookup(const std::string &key, Choice *c) {
     FILE *f = std::fopen(cache_path().c_str(), "r");
     if (!f) return false;
-    char kb[128];
-    int kk, ll;
-    double mm;
+    char line[256], kb[128];
     bool hit = false;
-    while (std::fscanf(f, "%127s %d %d %lf", kb, &kk, &ll, &mm) == 4) {
-        if (key == kb) {  // last entry wins
-            *k = kk;
-            *lpq = ll;
-            *ms = mm;
+    while (std::fgets(line, sizeof(line), f)) {
+        Choice x;
+        if (std::sscanf(line, "%127s %d %d %d %lf", kb, &x.k, &x.lpq, &x.mode, &x.ms) == 5 && key == kb) {
+            *c = x;
             hit = true;
         }
     }
@@ -81,7 +86,7 @@ bool cache_lookup(const std::string &key, int *k, int *lpq, double *ms) {
     return hit;
 }
 
-void cache_store(const std::string &key, int k, int lpq, double ms) {
+void cache_store(const std::string &key, const Choice &c) {
     const std::string path = cache_path();
     const size_t slash = path.find_last_of('/');
     if (slash != std::string::npos) {
@@ -91,26 +96,42 @@ void cache_store(const std::string &key, int k, int lpq, double ms) {
     }
     FILE *f = std::fopen(path.c_str(), "a");
     if (!f) return;  // read-only home: tuning still returns its result
-    std::fprintf(f, "%s %d %d %.6f\n", key.c_str(), k, lpq, ms);
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), c.k, c.lpq, c.mode, c.ms);
     std::fclose(f);
 }
 
 // Milliseconds per slide of `slides` timed slides (after one warm-up slide).
+// The stride points are on the device first and the slides are enqueued back
+// to back, as a serving loop runs them: the host copies of the synchronous
+// call (the whole window's labels, from pageable memory) would otherwise hide
+// the differences between the kernel forms.
 int trial(const rtdbs_params &p, const float *pts, int slides, double *ms) {
     rtdbs_stream *s = nullptr;
     int rc = rtdbs_stream_create(&p, &s);
     if (rc) return rc;
-    rc = rtdbs_stream_init(s, pts);
     const size_t W = (size_t)p.window, S = (size_t)p.stride, d = (size_t)p.dim;
-    std::vector<int32_t> label(W), cid(W);
-    if (!rc) rc = rtdbs_stream_slide(s, pts + W * d, label.data(), cid.data());
+    const size_t n_in = (size_t)(slides + 1) * S * d;
+    float *d_in = nullptr;
+    int32_t *d_label = nullptr, *d_cid = nullptr;
+    rc = rtdbs_stream_init(s, pts);
+    if (!rc && (hipMalloc((void **)&d_in, n_in * 4) != hipSuccess ||
+                hipMalloc((void **)&d_label, W * 4) != hipSuccess || hipMalloc((void **)&d_cid, W * 4) != hipSuccess))
+        rc = fail(RTDBS_ENOMEM, "autotune: trial buffers (%zu bytes)", (n_in + 2 * W) * 4);
+    if (!rc && hipMemcpy(d_in, pts + W * d, n_in * 4, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(RTDBS_EHIP, "autotune: uploading the sample failed");
+    if (!rc) rc = rtdbs_stream_slide_async(s, d_in, d_label, d_cid);
+    if (!rc) rc = rtdbs_stream_sync(s);
     if (!rc) {
         auto t0 = std::chrono::steady_clock::now();
         for (int t = 1; t <= slides && !rc; t++)
-            rc = rtdbs_stream_slide(s, pts + (W + (size_t)t * S) * d, label.data(), cid.data());
+            rc = rtdbs_stream_slide_async(s, d_in + (size_t)t * S * d, d_label, d_cid);
+        if (!rc) rc = rtdbs_stream_sync(s);
         auto t1 = std::chrono::steady_clock::now();
         *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / slides;
     }
+    if (d_in) (void)hipFree(d_in);
+    if (d_label) (void)hipFree(d_label);
+    if (d_cid) (void)hipFree(d_cid);
     rtdbs_stream_destroy(s);
     return rc;
 }
@@ -131,49 +152,55 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
     const std::string key = make_key(p0, points);
     std::snprintf(out->key, sizeof(out->key), "%s", key.c_str());
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    int k = 0, lpq = 0;
-    double ms = 0.0;
-    if (cache_lookup(key, &k, &lpq, &ms)) {
-        out->cell_div = k;
-        out->lanes_per_query = lpq;
-        out->ms_per_slide = ms;
+    Choice c;
+    if (cache_lookup(key, &c)) {
+        out->cell_div = c.k;
+        out->lanes_per_query = c.lpq;
+        out->neighbor_mode = c.mode;
+        out->ms_per_slide = c.ms;
         out->from_cache = 1;
         return RTDBS_OK;
     }
-    double best = 1e300;
-    int bk = 2, bl = 1;
+    // 1. stencil reach, with the launch-size defaults for the rest
+    Choice best;
+    best.ms = 1e300;
     for (int kk = 1; kk <= kMaxK; kk++) {
         rtdbs_params p = p0;
         p.cell_div = kk;
         p.lanes_per_query = 0;
+        p.neighbor_mode = 0;
         double m = 0.0;
         rc = trial(p, points, slides, &m);
         if (rc) return rc;
         out->trials++;
-        if (m < best) {
-            best = m;
-            bk = kk;
+        if (m < best.ms) {
+            best.ms = m;
+            best.k = kk;
         }
     }
-    // the automatic lanes-per-query choice of the k trials is one of these
-    // three, so the result always names an explicit setting
-    best = 1e300;
-    for (int ll : {1, 2, 4}) {
+    // 2. kernel form at that reach: cell-parallel, then point-parallel with
+    // 1, 2 and 4 lanes per query; the result always names one explicitly
+    best.ms = 1e300;
+    const int forms[4][2] = {{1, 0}, {2, 1}, {2, 2}, {2, 4}};
+    for (const auto &fm : forms) {
         rtdbs_params p = p0;
-        p.cell_div = bk;
-        p.lanes_per_query = ll;
+        p.cell_div = best.k;
+        p.neighbor_mode = fm[0];
+        p.lanes_per_query = fm[1];
         double m = 0.0;
         rc = trial(p, points, slides, &m);
         if (rc) return rc;
         out->trials++;
-        if (m < best) {
-            best = m;
-            bl = ll;
+        if (m < best.ms) {
+            best.ms = m;
+            best.mode = fm[0];
+            best.lpq = fm[1];
         }
     }
-    out->cell_div = bk;
-    out->lanes_per_query = bl;
-    out->ms_per_slide = best;
-    cache_store(key, bk, bl, best);
+    out->cell_div = best.k;
+    out->lanes_per_query = best.lpq;
+    out->neighbor_mode = best.mode;
+    out->ms_per_slide = best.ms;
+    cache_store(key, best);
     return RTDBS_OK;
 }

@@ -48,6 +48,8 @@ def test_version_and_error_string():
     (dict(eps=float("inf")), "eps"),
     (dict(min_pts=0), "min_pts"),
     (dict(cell_div=4), "cell_div"),
+    (dict(lanes_per_query=3), "lanes_per_query"),
+    (dict(neighbor_mode=3), "neighbor_mode"),
     (dict(window=(1 << 24) + 10, stride=10), "window"),
 ])
 def test_invalid_params_rejected(kw, msg):

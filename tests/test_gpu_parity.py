@@ -164,12 +164,16 @@ def test_autotune_choice_is_exact(engine, oracle, tmp_path, monkeypatch):
     W, S, slides = 4000, 400, 3
     data = _mixed(W + 8 * S, 3, 41)
     r = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
-    assert r["cell_div"] in (1, 2, 3) and r["lanes_per_query"] in (1, 2, 4)
-    assert not r["from_cache"] and r["trials"] == 6 and r["ms_per_slide"] > 0
+    assert r["cell_div"] in (1, 2, 3) and r["neighbor_mode"] in (1, 2)
+    assert r["lanes_per_query"] in ((0,) if r["neighbor_mode"] == 1 else (1, 2, 4))
+    assert not r["from_cache"] and r["trials"] == 7 and r["ms_per_slide"] > 0
     r2 = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
-    assert r2["from_cache"] and r2["cell_div"] == r["cell_div"] and r2["trials"] == 0
+    assert r2["from_cache"] and r2["trials"] == 0
+    assert (r2["cell_div"], r2["neighbor_mode"], r2["lanes_per_query"]) == \
+        (r["cell_div"], r["neighbor_mode"], r["lanes_per_query"])
     ref_l, ref_c = oracle.stream(data, W, S, 0.3, 6, 5, method="grid")
-    s = engine.Stream(3, W, S, 0.3, 6, cell_div=r["cell_div"], lanes_per_query=r["lanes_per_query"])
+    s = engine.Stream(3, W, S, 0.3, 6, cell_div=r["cell_div"], lanes_per_query=r["lanes_per_query"],
+                      neighbor_mode=r["neighbor_mode"])
     s.init(data[:W])
     for t in range(5):
         l, c = s.slide(data[W + t * S: W + (t + 1) * S])
