@@ -7,6 +7,28 @@ namespace rtdbs {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// A value every lane of the wave holds alike, moved to a scalar register (the
+// compiler cannot always prove it uniform; loops and addresses built from it
+// then stay scalar).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Global-memory view of a device pointer: per-lane accesses through it are
+// global loads and stores with a scalar base instead of flat accesses.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template <class T>
+__device__ __forceinline__ gptr<T> gl(T *p) {
+    return (gptr<T>)p;
+}
+
+// Read of a field of a descriptor that no kernel writes during a launch, as
+// constant memory: a scalar load when the address is uniform.
+template <class T>
+__device__ __forceinline__ T ldc(const T &f) {
+    return *(const __attribute__((address_space(4))) T *)&f;
+}
+
 // FP32 squared distance in the reference's order, never contracted:
 // util.cpp:310-313 / dbscan.cu:48-49 compute O.x*O.x + O.y*O.y + O.z*O.z.
 __device__ __forceinline__ float sqd(int dim, float ax, float ay, float az, float bx, float by,

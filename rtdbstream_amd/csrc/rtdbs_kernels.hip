@@ -1002,31 +1002,89 @@ __device__ __forceinline__ void stats_stream(const BatchArgs &a, WaveStats &w, i
     }
 }
 
-// The cell's stencil rows, one per lane (lane < R): start position `rx` and
-// exclusive prefix `pre` of the row lengths; returns the candidate total.
-__device__ __forceinline__ int cell_rows(const StreamDev &S, int A, int *rx, int *pre) {
-    const int lane = lane_id();
-    int x = 0, cnt = 0;
-    if (lane < S.R) {
-        const int2 r = S.rows[(size_t)A * S.R + lane];
-        x = r.x;
-        cnt = r.y - r.x;
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    *rx = x;
-    *pre = incl - cnt;
-    return __shfl(incl, 63);
+// Wave-uniform view of one stream for the cell-parallel kernels. The
+// descriptor is read with scalar loads (no kernel writes it during a launch)
+// and the arrays are addressed as global memory, so the per-lane accesses
+// are global loads and stores off a scalar base.
+struct CellView {
+    gptr<const float> sx, sy, sz;
+    gptr<const uint64_t> skey;
+    gptr<const int32_t> cstart;
+    gptr<const uint64_t> rows;  // int2 (first, end) per stencil row, as one word
+    gptr<const uint8_t> cflag;
+    gptr<uint8_t> flags;
+    gptr<int32_t> fcore, parent, canon, rep;
+    int R, dim, min_pts;
+    uint32_t smask;
+    float eps2;
+};
+
+__device__ __forceinline__ CellView cell_view(const StreamDev &S) {
+    CellView v;
+    v.sx = gl(ldc(S.sx));
+    v.sy = gl(ldc(S.sy));
+    v.sz = gl(ldc(S.sz));
+    v.skey = gl(ldc(S.skey));
+    v.cstart = gl(ldc(S.cstart));
+    v.rows = gl((const uint64_t *)ldc(S.rows));
+    v.cflag = gl(ldc(S.cflag));
+    v.flags = gl(ldc(S.flags));
+    v.fcore = gl(ldc(S.fcore));
+    v.parent = gl(ldc(S.parent));
+    v.canon = gl(ldc(S.canon));
+    v.rep = gl(ldc(S.rep));
+    v.R = ldc(S.R);
+    v.dim = ldc(S.dim);
+    v.min_pts = ldc(S.min_pts);
+    v.smask = (uint32_t)((1ull << ldc(S.slot_bits)) - 1ull);
+    v.eps2 = ldc(S.eps2);
+    return v;
 }
 
-// Position of flat candidate j (< total) of the cell's rows.
-__device__ __forceinline__ int cell_candidate(int rx, int pre, int R, int j) {
-    const int r = row_of(pre, R, j);
-    return __shfl(rx, r) + (j - __shfl(pre, r));
+__device__ __forceinline__ int vslot(const CellView &V, int pos) { return (int)((uint32_t)V.skey[pos] & V.smask); }
+
+// A cell's stencil candidates, 64 at a time. Lane r < R holds stencil row r:
+// its first window position and its length. A walk in scalar registers opens
+// the rows in order (reading each with readlane) and gives every lane the
+// position of its candidate, which belongs to the last row opened at or
+// before it - no cross-lane search per chunk.
+struct RowWalk {
+    int rx = 0, len = 0;  // lane r: row r
+    int r = 0, acc = 0;   // next row to open and its first candidate (uniform)
+    int base = 0;         // position minus candidate index in the last row opened (uniform)
+};
+
+// Rows of cell A, each cut at window position `limit`.
+__device__ __forceinline__ RowWalk row_walk(const CellView &V, int A, int limit) {
+    RowWalk w;
+    const int lane = lane_id();
+    if (lane < V.R) {
+        const uint64_t r = V.rows[(size_t)A * V.R + lane];
+        w.rx = (int)(uint32_t)r;
+        w.len = max(0, min((int)(uint32_t)(r >> 32), limit) - w.rx);
+    }
+    return w;
+}
+
+// Opens the rows that start before jb + 64 and returns this lane's candidate
+// position in chunk [jb, jb + 64); lanes past the last candidate get the last
+// one. *end is the candidate count once every row is open, else INT_MAX.
+// Callers stop at jb >= *end before using the position.
+__device__ __forceinline__ int walk_chunk(RowWalk &w, int R, int jb, int *end) {
+    const int j = jb + lane_id();
+    int base = w.base;
+    while (w.r < R && w.acc < jb + 64) {
+        const int len = __builtin_amdgcn_readlane(w.len, w.r);
+        if (len > 0) {
+            const int b = __builtin_amdgcn_readlane(w.rx, w.r) - w.acc;
+            if (j >= w.acc) base = b;
+            w.base = b;
+            w.acc += len;
+        }
+        w.r++;
+    }
+    *end = w.r < R ? INT_MAX : w.acc;
+    return base + min(j, *end - 1);
 }
 
 // Range count = core identification (the reference's __raygen__grid +
@@ -1034,26 +1092,25 @@ __device__ __forceinline__ int cell_candidate(int rx, int pre, int R, int j) {
 // aabb.cu:575-587). Dense cells are core without counting (all pairs within
 // eps). Otherwise each query counts its neighbours, itself included, and
 // stops once the count reaches min_pts.
-__device__ void count_cell(const StreamDev &S, int A, WaveStats &st) {
+__device__ void count_cell(const CellView &V, int A, WaveStats &st) {
     const int lane = lane_id();
-    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
-    const uint8_t cf = S.cflag[A];
+    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
+    const int cf = uni(V.cflag[A]);
     if (cf & kCellDense) {
         st.v[kStatDensePoints] += nA;
+        const int rs = uni(vslot(V, cs));
         for (int i = lane; i < nA; i += 64) {
-            const int p = cs + i, sl = slot_of(S, p);
-            S.parent[sl] = sl;
-            S.flags[p] = kFlagCore | kFlagDense;
+            const int p = cs + i, sl = vslot(V, p);
+            V.parent[sl] = sl;
+            V.flags[p] = kFlagCore | kFlagDense;
+            V.rep[p] = rs;
         }
-        const int rs = slot_of(S, cs);
-        for (int i = lane; i < nA; i += 64) S.rep[cs + i] = rs;
-        if (lane == 0) S.fcore[A] = cs;
+        if (lane == 0) V.fcore[A] = cs;
         return;
     }
-    const int R = S.R, dim = S.dim, minpts = S.min_pts;
-    const float e2 = S.eps2;
-    int rx, pre;
-    const int TC = cell_rows(S, A, &rx, &pre);
+    const int R = V.R, dim = V.dim, minpts = V.min_pts;
+    const float e2 = V.eps2;
+    const RowWalk w0 = row_walk(V, A, INT_MAX);
     int fc = kNoCore;
     st.v[kStatQueries] += nA;
     for (int q0 = 0; q0 < nA; q0 += 64) {
@@ -1061,21 +1118,23 @@ __device__ void count_cell(const StreamDev &S, int A, WaveStats &st) {
         const int pq = cs + q0 + lane;
         float qx = 0.f, qy = 0.f, qz = 0.f;
         if (lane < nq) {
-            qx = S.sx[pq];
-            if (dim > 1) qy = S.sy[pq];
-            if (dim > 2) qz = S.sz[pq];
+            qx = V.sx[pq];
+            if (dim > 1) qy = V.sy[pq];
+            if (dim > 2) qz = V.sz[pq];
         }
         int mine = 0;
         uint64_t open = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
-        for (int jb = 0; jb < TC && open; jb += 64) {
-            const int j = jb + lane;
-            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
-            const float cx = S.sx[pos];
-            const float cy = dim > 1 ? S.sy[pos] : 0.f;
-            const float cz = dim > 2 ? S.sz[pos] : 0.f;
-            const bool v = j < TC;
+        RowWalk w = w0;
+        for (int jb = 0; open; jb += 64) {
+            int end;
+            const int pos = walk_chunk(w, R, jb, &end);
+            if (jb >= end) break;
+            const float cx = V.sx[pos];
+            const float cy = dim > 1 ? V.sy[pos] : 0.f;
+            const float cz = dim > 2 ? V.sz[pos] : 0.f;
+            const bool v = jb + lane < end;
             uint64_t m = open;
-            st.v[kStatCountTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, TC - jb);
+            st.v[kStatCountTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb);
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1086,29 +1145,30 @@ __device__ void count_cell(const StreamDev &S, int A, WaveStats &st) {
             open &= ~__ballot(mine >= minpts);
         }
         if (lane < nq) {
-            const int sl = slot_of(S, pq);
-            S.parent[sl] = sl;
-            S.flags[pq] = mine >= minpts ? kFlagCore : 0;
+            const int sl = vslot(V, pq);
+            V.parent[sl] = sl;
+            V.flags[pq] = mine >= minpts ? kFlagCore : 0;
         }
         const uint64_t cm = __ballot(lane < nq && mine >= minpts);
         if (fc == kNoCore && cm != 0ull) fc = cs + q0 + __ffsll((unsigned long long)cm) - 1;
     }
-    if (lane == 0) S.fcore[A] = fc;  // first (minimum-slot) core: the cell's representative
+    if (lane == 0) V.fcore[A] = fc;  // first (minimum-slot) core: the cell's representative
     // union-find element of each point: the representative's slot in a safe
     // cell, else its own slot
     const bool safe = cf & kCellSafe;
-    const int rs = fc != kNoCore ? slot_of(S, fc) : 0;
-    for (int i = lane; i < nA; i += 64) S.rep[cs + i] = safe ? rs : slot_of(S, cs + i);
+    const int rs = fc != kNoCore ? uni(vslot(V, fc)) : 0;
+    for (int i = lane; i < nA; i += 64) V.rep[cs + i] = safe ? rs : vslot(V, cs + i);
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
-    const int total = a.cell_pre[a.n];
+    const int total = uni(a.cell_pre[a.n]);
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
-    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+    for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        stats_stream(a, st, x.s);
-        count_cell(a.s[x.s], x.c, st);
+        const int s = uni(x.s);
+        stats_stream(a, st, s);
+        count_cell(cell_view(a.s[s]), uni(x.c), st);
     }
     stats_flush(a, st);
 }
@@ -1116,37 +1176,44 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
 // deterministic rule: the minimum canonical id among cores within eps) for
 // the non-core points of a cell; cells without them are skipped.
-__device__ void border_cell(const StreamDev &S, const IO &io, int A, WaveStats &st) {
+__device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &st) {
     const int lane = lane_id();
-    if (S.cflag[A] & kCellDense) return;
-    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
-    const int R = S.R, dim = S.dim;
-    const float e2 = S.eps2;
-    int rx = 0, pre = 0, TC = -1;
+    if (uni(V.cflag[A]) & kCellDense) return;
+    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
+    const int R = V.R, dim = V.dim;
+    const float e2 = V.eps2;
+    gptr<int32_t> label = gl(io.label), cidv = gl(io.cid);
+    RowWalk w0;
+    bool have_rows = false;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + lane;
-        const bool isq = lane < nq && !(S.flags[pq] & kFlagCore);
+        const bool isq = lane < nq && !(V.flags[pq] & kFlagCore);
         const uint64_t open = __ballot(isq);
         if (open == 0ull) continue;
-        if (TC < 0) TC = cell_rows(S, A, &rx, &pre);
+        if (!have_rows) {
+            w0 = row_walk(V, A, INT_MAX);
+            have_rows = true;
+        }
         float qx = 0.f, qy = 0.f, qz = 0.f;
         if (isq) {
-            qx = S.sx[pq];
-            if (dim > 1) qy = S.sy[pq];
-            if (dim > 2) qz = S.sz[pq];
+            qx = V.sx[pq];
+            if (dim > 1) qy = V.sy[pq];
+            if (dim > 2) qz = V.sz[pq];
         }
         int best = INT_MAX;
-        for (int jb = 0; jb < TC; jb += 64) {
-            const int j = jb + lane;
-            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
-            const float cx = S.sx[pos];
-            const float cy = dim > 1 ? S.sy[pos] : 0.f;
-            const float cz = dim > 2 ? S.sz[pos] : 0.f;
-            const bool core = j < TC && (S.flags[pos] & kFlagCore);
-            const int cid = core ? S.canon[pos] : INT_MAX;
+        RowWalk w = w0;
+        for (int jb = 0;; jb += 64) {
+            int end;
+            const int pos = walk_chunk(w, R, jb, &end);
+            if (jb >= end) break;
+            const float cx = V.sx[pos];
+            const float cy = dim > 1 ? V.sy[pos] : 0.f;
+            const float cz = dim > 2 ? V.sz[pos] : 0.f;
+            const bool core = jb + lane < end && (V.flags[pos] & kFlagCore);
+            const int cid = core ? V.canon[pos] : INT_MAX;
             uint64_t m = open;
-            st.v[kStatBorderTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, TC - jb);
+            st.v[kStatBorderTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb);
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1157,21 +1224,22 @@ __device__ void border_cell(const StreamDev &S, const IO &io, int A, WaveStats &
             }
         }
         if (isq) {
-            const int slot = slot_of(S, pq);
-            io.label[slot] = best != INT_MAX ? 1 : 2;
-            io.cid[slot] = best != INT_MAX ? best : slot;
+            const int slot = vslot(V, pq);
+            label[slot] = best != INT_MAX ? 1 : 2;
+            cidv[slot] = best != INT_MAX ? best : slot;
         }
     }
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
-    const int total = a.cell_pre[a.n];
+    const int total = uni(a.cell_pre[a.n]);
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
-    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+    for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        stats_stream(a, st, x.s);
-        border_cell(a.s[x.s], a.use_io1 ? a.io1 : a.io[x.s], x.c, st);
+        const int s = uni(x.s);
+        stats_stream(a, st, s);
+        border_cell(cell_view(a.s[s]), a.use_io1 ? a.io1 : a.io[s], uni(x.c), st);
     }
     stats_flush(a, st);
 }
@@ -1223,62 +1291,49 @@ __device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int r
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
-__device__ void union_cell(const StreamDev &S, int A, WaveStats &st, EdgeBuf &eb) {
+__device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
     const int lane = lane_id();
-    const int fcA = S.fcore[A];
+    const int fcA = uni(V.fcore[A]);
     if (fcA == kNoCore) return;
-    const uint8_t cfA = S.cflag[A];
+    const int cfA = uni(V.cflag[A]);
     const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
-    const int cs = S.cstart[A], nA = S.cstart[A + 1] - cs;
-    const int R = S.R, dim = S.dim;
-    const float e2 = S.eps2;
-    const int limit = safeA ? cs : cs + nA;
-    int rx = 0, pre = 0, cnt = 0;
-    if (lane < R) {
-        const int2 r = S.rows[(size_t)A * R + lane];
-        rx = r.x;
-        cnt = max(0, min(r.y, limit) - r.x);
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    pre = incl - cnt;
-    const int TC = __shfl(incl, 63);
-    if (TC == 0) return;
-    const int repA = slot_of(S, fcA);
+    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
+    const int R = V.R, dim = V.dim;
+    const float e2 = V.eps2;
+    const RowWalk w0 = row_walk(V, A, safeA ? cs : cs + nA);
+    if (__ballot(w0.len > 0) == 0ull) return;
+    const int repA = uni(vslot(V, fcA));
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + lane;
-        const bool qcore = lane < nq && (denseA || (S.flags[pq] & kFlagCore));
+        const bool qcore = lane < nq && (denseA || (V.flags[pq] & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
         float qx = 0.f, qy = 0.f, qz = 0.f;
         int qslot = 0;
         if (qcore) {
-            qx = S.sx[pq];
-            if (dim > 1) qy = S.sy[pq];
-            if (dim > 2) qz = S.sz[pq];
-            qslot = slot_of(S, pq);
+            qx = V.sx[pq];
+            if (dim > 1) qy = V.sy[pq];
+            if (dim > 2) qz = V.sz[pq];
+            qslot = vslot(V, pq);
         }
-        for (int jb = 0; jb < TC; jb += 64) {
-            const int j = jb + lane;
-            const int pos = cell_candidate(rx, pre, R, j < TC ? j : TC - 1);
-            const bool bcore = j < TC && (S.flags[pos] & kFlagCore);
+        RowWalk w = w0;
+        for (int jb = 0;; jb += 64) {
+            int end;
+            const int pos = walk_chunk(w, R, jb, &end);
+            if (jb >= end) break;
+            const bool bcore = jb + lane < end && (V.flags[pos] & kFlagCore);
             const uint64_t bm = __ballot(bcore);
             if (bm == 0ull) continue;
-            const float cx = S.sx[pos];
-            const float cy = dim > 1 ? S.sy[pos] : 0.f;
-            const float cz = dim > 2 ? S.sz[pos] : 0.f;
-            const int rep = bcore ? S.rep[pos] : 0;
-            const unsigned chunk = (unsigned)min(64, TC - jb);
+            const float cx = V.sx[pos];
+            const float cy = dim > 1 ? V.sy[pos] : 0.f;
+            const float cz = dim > 2 ? V.sz[pos] : 0.f;
+            const int rep = bcore ? V.rep[pos] : 0;
+            const unsigned chunk = (unsigned)min(64, end - jb);
             uint64_t m = qm;
             if (safeA) {
                 // lanes holding one element form a run (a safe cell's points are
                 // adjacent); one touching lane settles its whole run
-                const int lane = lane_id();
                 const int prev_rep = __shfl_up(rep, 1);  // every lane takes part in the shuffles
                 const int prev_core = __shfl_up((int)bcore, 1);
                 const uint64_t starts = __ballot(lane == 0 || prev_rep != rep || !prev_core);
@@ -1314,20 +1369,21 @@ __device__ void union_cell(const StreamDev &S, int A, WaveStats &st, EdgeBuf &eb
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     __shared__ int2 edges[kCellWaves][kEdgeCap];
-    const int total = a.cell_pre[a.n];
+    const int total = uni(a.cell_pre[a.n]);
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0, nullptr};
     int es = -1;  // stream of the queued pairs
-    for (int gc = blockIdx.x * kCellWaves + (threadIdx.x >> 6); gc < total; gc += nw) {
+    for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
-        stats_stream(a, st, x.s);
-        if (x.s != es) {
+        const int s = uni(x.s);
+        stats_stream(a, st, s);
+        if (s != es) {
             if (es >= 0) edge_flush(eb);
-            es = x.s;
-            eb.parent = a.s[x.s].parent;
+            es = s;
+            eb.parent = ldc(a.s[s].parent);
         }
-        union_cell(a.s[x.s], x.c, st, eb);
+        union_cell(cell_view(a.s[s]), uni(x.c), st, eb);
     }
     if (es >= 0) edge_flush(eb);
     stats_flush(a, st);
