@@ -20,7 +20,6 @@ constexpr int kMaxK = 3;           // largest stencil reach
 constexpr int kMaxRows = 49;       // (2k+1)^(dim-1) for k <= 3, dim <= 3
 constexpr int kKeyBits = 40;       // cell key bits; slot bits = 64 - 40 at most 24
 constexpr int kMaxSlotBits = 24;
-constexpr uint32_t kEpochMax = (1u << 24) - 1;  // hash epochs cycle 1..kEpochMax
 constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
 // Per-stream work counters (optional; rtdbs_stats in the C ABI)
@@ -46,9 +45,8 @@ constexpr int kNoCore = 0x7fffffff;  // fcore of a cell without cores
 struct StreamMut {
     int32_t t;       // slides completed
     int32_t u;       // ring unit the next stride overwrites: slots [u*S, u*S+S)
-    uint32_t epoch;  // cell-hash epoch of the next slide
     int32_t ncell;   // non-empty cells of the current window
-    int32_t pad[4];
+    int32_t pad[5];
 };
 
 // Per-slide I/O (device pointers).
@@ -63,8 +61,6 @@ struct StreamDev {
     int32_t k, R, slot_bits, units;
     int32_t dense_ok;  // side eps/k with k*k > dim: points of one interior cell are within eps
     float eps2;        // fl32(eps*eps)
-    uint32_t hmask;    // hash table size - 1
-    int32_t pad0;
     double inv_side;
     double lo[3];
     int64_t n[3];  // grid cells per axis (1 for unused axes)
@@ -84,8 +80,6 @@ struct StreamDev {
     int32_t *cstart;  // W+1
     uint64_t *ckey;   // W
     int32_t *pcell;   // W: position -> cell
-    uint64_t *htab;   // cell hash: (epoch << 40) | cell_key
-    int32_t *hval;    // cell index
     int2 *rows;       // W*R: candidate position ranges per cell
     uint8_t *cflag;   // W: kCellSafe | kCellDense per cell
     int32_t *fcore;   // W: first (minimum-slot) core position of each cell, kNoCore if none
@@ -149,7 +143,5 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
 hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipStream_t st);
 // Copies n16 16-byte words (profiler byte-counter calibration).
 hipError_t launch_copy16(const void *src, void *dst, size_t n16, hipStream_t st);
-// Clears the cell hash of every stream in the batch (epoch wrap).
-hipError_t launch_clear_hash(const BatchArgs &a, hipStream_t st);
 
 }  // namespace rtdbs

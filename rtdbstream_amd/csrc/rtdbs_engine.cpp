@@ -116,12 +116,12 @@ int validate_params(const rtdbs_params *p) {
 struct Layout {
     size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
     size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
-    size_t off_cstart, off_ckey, off_pcell, off_htab, off_hval, off_rows, off_cflag, off_fcore, off_flags, off_parent,
-        off_canon, off_rep, off_tile_a, off_tile_b, off_in, off_label, off_cid;
+    size_t off_cstart, off_ckey, off_pcell, off_rows, off_cflag, off_fcore, off_flags, off_parent, off_canon, off_rep,
+        off_tile_a, off_tile_b, off_in, off_label, off_cid;
     size_t total;
 };
 
-static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
+static Layout make_layout(const rtdbs_params &p, int R) {
     Layout L;
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -151,8 +151,6 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
     L.off_cstart = take((W + 1) * 4);
     L.off_ckey = take(W * 8);
     L.off_pcell = take(W * 4);
-    L.off_htab = take((size_t)hsize * 8);
-    L.off_hval = take((size_t)hsize * 4);
     L.off_rows = take(W * (size_t)R * 8);
     L.off_cflag = take(W);
     L.off_fcore = take(W * 4);
@@ -169,19 +167,17 @@ static Layout make_layout(const rtdbs_params &p, int R, uint32_t hsize) {
     return L;
 }
 
-static void stream_shape(const rtdbs_params &p, int *k, int *R, uint32_t *hsize) {
+static void stream_shape(const rtdbs_params &p, int *k, int *R) {
     *k = p.cell_div > 0 ? p.cell_div : default_cell_div(p);
     int r = 1;
     for (int a = 0; a < p.dim - 1; a++) r *= 2 * (*k) + 1;
     *R = r;
-    *hsize = next_pow2(2ull * (uint64_t)p.window);
 }
 
 size_t stream_device_bytes(const rtdbs_params &p) {
     int k, R;
-    uint32_t hs;
-    stream_shape(p, &k, &R, &hs);
-    return make_layout(p, R, hs).total;
+    stream_shape(p, &k, &R);
+    return make_layout(p, R).total;
 }
 
 }  // namespace rtdbs
@@ -205,10 +201,10 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     rtdbs_stream *s = new (std::nothrow) rtdbs_stream();
     if (!s) return fail(RTDBS_ENOMEM, "host allocation failed");
     s->p = *params;
-    stream_shape(s->p, &s->k, &s->R, &s->hsize);
+    stream_shape(s->p, &s->k, &s->R);
     s->dense_ok = (s->k * s->k > s->p.dim) ? 1 : 0;
     s->slot_bits = std::max(1, ceil_log2(s->p.window));
-    const Layout L = make_layout(s->p, s->R, s->hsize);
+    const Layout L = make_layout(s->p, s->R);
     s->bytes = L.total;
     if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
     if (alloc_hook_should_fail(L.total)) {
@@ -221,7 +217,7 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
         (void)hipGetLastError();
         return fail(RTDBS_ENOMEM, "hipMalloc(%zu) failed: %s", L.total, hipGetErrorString(e));
     }
-    // zeroed hash (epoch 0 entries are stale for every epoch >= 1) and state
+    // zeroed state
     e = hipMemset(s->arena, 0, L.total);
     if (e != hipSuccess) {
         (void)hipFree(s->arena);
@@ -244,7 +240,6 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
         volatile float e_ = s->p.eps;
         D.eps2 = e_ * e_;  // fl32(eps*eps), dbscan.cpp:41
     }
-    D.hmask = s->hsize - 1;
     D.mut = (StreamMut *)(A + L.off_mut);
     auto f = [&](size_t off, bool used) { return used ? (float *)(A + off) : (float *)nullptr; };
     const int d = s->p.dim;
@@ -267,8 +262,6 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     D.cstart = (int32_t *)(A + L.off_cstart);
     D.ckey = (uint64_t *)(A + L.off_ckey);
     D.pcell = (int32_t *)(A + L.off_pcell);
-    D.htab = (uint64_t *)(A + L.off_htab);
-    D.hval = (int32_t *)(A + L.off_hval);
     D.rows = (int2 *)(A + L.off_rows);
     D.cflag = (uint8_t *)(A + L.off_cflag);
     D.fcore = (int32_t *)(A + L.off_fcore);
@@ -358,9 +351,8 @@ static int stream_load(rtdbs_stream *s, const float *points, int64_t slides_done
     const size_t W = (size_t)s->p.window, d = (size_t)s->p.dim;
     const int units = s->p.window / s->p.stride;
     HIPCHK(hipMemcpyAsync(s->d_in, points, W * d * 4, hipMemcpyHostToDevice, b->hs));
-    StreamMut m0 = {(int32_t)(slides_done % (1 << 30)), (int32_t)(slides_done % units), 1u, 0, {0, 0, 0, 0}};
+    StreamMut m0 = {(int32_t)(slides_done % (1 << 30)), (int32_t)(slides_done % units), 0, {0, 0, 0, 0, 0}};
     HIPCHK(hipMemcpyAsync(s->dev.mut, &m0, sizeof(m0), hipMemcpyHostToDevice, b->hs));
-    HIPCHK(hipMemsetAsync(s->dev.htab, 0, (size_t)s->hsize * 8, b->hs));
     BatchArgs a = b->args;
     a.use_io1 = 1;
     a.io1.in = s->d_in;
@@ -373,7 +365,6 @@ static int stream_load(rtdbs_stream *s, const float *points, int64_t slides_done
     HIPCHK(hipStreamSynchronize(b->hs));
     s->initialized = true;
     s->slides = slides_done;
-    s->epoch_base = slides_done;
     return RTDBS_OK;
 }
 
@@ -649,21 +640,6 @@ static int batch_check_current(rtdbs_batch *b) {
     return RTDBS_OK;
 }
 
-// Clears the cell hash of streams whose epoch is about to wrap.
-static int batch_epoch_wrap(rtdbs_batch *b) {
-    for (int i = 0; i < b->n; i++) {
-        rtdbs_stream *s = b->streams[i];
-        const int64_t since = s->slides - s->epoch_base;
-        if (since > 0 && since % (int64_t)kEpochMax == 0) {
-            int rc = ensure_solo(s);
-            if (rc) return rc;
-            BatchArgs a = s->solo->args;
-            HIPCHK(launch_clear_hash(a, b->hs));
-        }
-    }
-    return RTDBS_OK;
-}
-
 int ensure_solo(rtdbs_stream *s) {
     if (s->solo) return RTDBS_OK;
     int rc = 0;
@@ -684,8 +660,6 @@ static void timing_begin(rtdbs_batch *b) { (void)b; }
 
 int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *cid) {
     int rc = batch_check_current(b);
-    if (rc) return rc;
-    rc = batch_epoch_wrap(b);
     if (rc) return rc;
     BatchArgs a = b->args;
     a.use_io1 = 1;
@@ -730,8 +704,6 @@ int batch_sync(rtdbs_batch *b) {
 // One slide of all member streams with device I/O (table in d_io).
 int batch_launch(rtdbs_batch *b, const IO *io) {
     int rc = batch_check_current(b);
-    if (rc) return rc;
-    rc = batch_epoch_wrap(b);
     if (rc) return rc;
     bool changed = !b->io_valid;
     for (int i = 0; i < b->n && !changed; i++)

@@ -13,7 +13,6 @@ struct rtdbs_batch;
 struct rtdbs_stream {
     rtdbs_params p{};
     int k = 0, R = 0, slot_bits = 0, dense_ok = 0;
-    uint32_t hsize = 0;
     double side = 0.0;
     double lo[3] = {0, 0, 0};
     int64_t n[3] = {1, 1, 1};
@@ -25,7 +24,6 @@ struct rtdbs_stream {
     int32_t *d_cid = nullptr;
     bool initialized = false;
     int64_t slides = 0;
-    int64_t epoch_base = 0;    // value of `slides` when the cell hash was last cleared
     uint64_t gen = 0;          // bumped by every init: batches re-read descriptors
     rtdbs_batch *solo = nullptr;  // single-stream batch of the plain path
     int device = 0;

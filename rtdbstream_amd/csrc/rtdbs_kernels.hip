@@ -151,15 +151,6 @@ __device__ __forceinline__ uint64_t cell_key_of(const StreamDev &S, const float 
     return (uint64_t)((ix[0] * S.n[1] + ix[1]) * S.n[2] + ix[2]);
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-    x ^= x >> 33;
-    x *= 0xff51afd7ed558ccdULL;
-    x ^= x >> 33;
-    x *= 0xc4ceb9fe1a85ec53ULL;
-    x ^= x >> 33;
-    return x;
-}
-
 // Quotient and remainder of a cell key (< 2^40) by a grid extent (< 2^31),
 // via a double-precision estimate and an exact integer fix-up (64-bit integer
 // division is a long software sequence on the GPU).
@@ -176,47 +167,6 @@ __device__ __forceinline__ void divmod_key(uint64_t x, int64_t d, int64_t *q, in
     }
     *q = qq;
     *r = rr;
-}
-
-__device__ __forceinline__ void decode_key(const StreamDev &S, uint64_t ck, int64_t ix[3]) {
-    int64_t q, r;
-    divmod_key(ck, S.n[2], &q, &r);
-    ix[2] = r;
-    int64_t q2, r2;
-    divmod_key((uint64_t)q, S.n[1], &q2, &r2);
-    ix[1] = r2;
-    ix[0] = q2;
-}
-
-// Open-addressing cell hash; an entry whose epoch differs from the current
-// one counts as empty, so the table never needs clearing between slides.
-__device__ __forceinline__ void hash_insert(const StreamDev &S, uint64_t ck, int c, uint32_t epoch) {
-    const uint64_t tag = ((uint64_t)epoch << kKeyBits) | ck;
-    uint32_t h = (uint32_t)mix64(ck) & S.hmask;
-    while (true) {
-        uint64_t cur = __hip_atomic_load(S.htab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)(cur >> kKeyBits) != epoch) {
-            uint64_t prev = atomicCAS((unsigned long long *)(S.htab + h), (unsigned long long)cur,
-                                      (unsigned long long)tag);
-            if (prev == cur) {
-                S.hval[h] = c;
-                return;
-            }
-            continue;
-        }
-        h = (h + 1) & S.hmask;
-    }
-}
-
-__device__ __forceinline__ int hash_find(const StreamDev &S, uint64_t ck, uint32_t epoch) {
-    const uint64_t tag = ((uint64_t)epoch << kKeyBits) | ck;
-    uint32_t h = (uint32_t)mix64(ck) & S.hmask;
-    while (true) {
-        uint64_t cur = S.htab[h];
-        if (cur == tag) return S.hval[h];
-        if ((uint32_t)(cur >> kKeyBits) != epoch) return -1;
-        h = (h + 1) & S.hmask;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -668,21 +618,18 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
     if (threadIdx.x == 0) S.tile_b[tile] = starts;
 }
 
-// Cell table: ordered compaction of run starts -> cstart/ckey, position ->
-// cell, and the cell hash. Replaces compute_offsets_of_cells + thrust_sort
+// Cell table: ordered compaction of run starts -> cstart/ckey and position ->
+// cell. Replaces compute_offsets_of_cells + thrust_sort
 // (aabb.cu:377-410, thrust_helper.cu:16-19): the order comes from the scan,
 // so no sort and no host round trip (dbscan.cpp:549) are needed.
 __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     __shared__ int lds4[4];
-    __shared__ int newc[kWTile];        // this tile's new cells ...
-    __shared__ uint64_t newk[kWTile];   // ... and their keys
     const int g = blockIdx.x;
     const int s = find_stream(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
     const int tile = g - a.pre_wtile[s];
     const int ntile = a.pre_wtile[s + 1] - a.pre_wtile[s];
     const int p0 = tile * kWTile;
-    const uint32_t epoch = S.mut->epoch;
     const int sb = S.slot_bits;
     constexpr int per = kWTile / kTileThreads;
     uint64_t kc[per], kp[per];  // keys of each position and its predecessor, loaded up front
@@ -713,15 +660,8 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
             const int c = base + run + rank;
             S.cstart[c] = p;
             S.ckey[c] = ck;
-            newc[run + rank] = c;
-            newk[run + rank] = ck;
         }
         run += tot;
-    }
-    __syncthreads();
-    // hash inserts after the scan, spread over the block
-    for (int i = threadIdx.x; i < run; i += kTileThreads) {
-        hash_insert(S, newk[i], newc[i], epoch);
     }
     if (tile == ntile - 1 && threadIdx.x == 0) {
         const int total = base + run;
@@ -733,58 +673,98 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // stencil rows: for every cell and every row of its (2k+1)^dim stencil, the
 // contiguous range of window positions holding that row's cells. A row varies
-// only the fastest axis, so its cells are consecutive in the sorted table.
+// the last used axis, whose key stride is 1 (unused axes have extent 1 and
+// index 0), so its cells are the cells with keys in [lo, hi] = the target
+// column's keys at zlo..zhi, consecutive in the sorted cell table. For one
+// row offset, lo and hi rise with the cell's own key (the target column is
+// the cell's column shifted by a constant), so one wave serves 64 consecutive
+// cells: a cooperative search brackets the tile's bounds in the cell table,
+// the bracket is staged in LDS and each lane binary-searches its own bounds
+// there. No hashing, and a few memory round trips per 64 (cell, row) items.
 
-__device__ __forceinline__ int2 cell_row(const StreamDev &S, int c, int r, uint32_t epoch) {
-    int64_t ix[3];
-    decode_key(S, S.ckey[c], ix);
-    const int k = S.k, last = S.dim - 1, w = 2 * k + 1;
-    const int center = (S.R - 1) / 2;
-    int rr = r == 0 ? center : (r == center ? 0 : r);  // centre row first
-    int64_t jx[3] = {ix[0], ix[1], ix[2]};
-    for (int ax = S.dim - 2; ax >= 0; ax--) {
-        const int off = rr % w - k;
-        rr /= w;
-        jx[ax] = ix[ax] + off;
-        if (jx[ax] < 0 || jx[ax] >= S.n[ax]) return make_int2(0, 0);
-    }
-    const int64_t zlo = ix[last] - k < 0 ? 0 : ix[last] - k;
-    const int64_t zhi = ix[last] + k >= S.n[last] ? S.n[last] - 1 : ix[last] + k;
-    const int nz = (int)(zhi - zlo + 1);
-    // first probe of every cell of the row, issued together
-    uint64_t tag[2 * kMaxK + 1];
-    uint32_t h[2 * kMaxK + 1];
-    uint64_t cur[2 * kMaxK + 1];
-#pragma unroll
-    for (int i = 0; i < 2 * kMaxK + 1; i++) {
-        if (i < nz) {
-            jx[last] = zlo + i;
-            const uint64_t key = (uint64_t)((jx[0] * S.n[1] + jx[1]) * S.n[2] + jx[2]);
-            tag[i] = ((uint64_t)epoch << kKeyBits) | key;
-            h[i] = (uint32_t)mix64(key) & S.hmask;
-            cur[i] = S.htab[h[i]];
+constexpr int kRowTile = 64;    // cells per row-kernel item (one wave)
+constexpr int kRowCap = 192;    // bracket cells staged in LDS per wave
+constexpr int kRowWaves = kTileThreads / 64;
+
+// Wave-uniform view of one stream for the row kernel (scalar descriptor
+// loads, global addressing).
+struct RowsView {
+    gptr<const uint64_t> ckey;
+    gptr<const int32_t> cstart;
+    gptr<uint64_t> rows;  // int2 (first, end) per stencil row, as one word
+    gptr<uint8_t> cflag;
+    gptr<int32_t> fcore;
+    int64_t n0, n1, n2;
+    int k, R, dim, min_pts, dense_ok, ncell;
+};
+
+__device__ __forceinline__ RowsView rows_view(const StreamDev &S) {
+    RowsView v;
+    v.ckey = gl(ldc(S.ckey));
+    v.cstart = gl(ldc(S.cstart));
+    v.rows = gl((uint64_t *)ldc(S.rows));
+    v.cflag = gl(ldc(S.cflag));
+    v.fcore = gl(ldc(S.fcore));
+    v.n0 = ldc(S.n[0]);
+    v.n1 = ldc(S.n[1]);
+    v.n2 = ldc(S.n[2]);
+    v.k = ldc(S.k);
+    v.R = ldc(S.R);
+    v.dim = ldc(S.dim);
+    v.min_pts = ldc(S.min_pts);
+    v.dense_ok = ldc(S.dense_ok);
+    v.ncell = ldc(ldc(S.mut)->ncell);  // written by the cell-table kernel
+    return v;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Ranks (number of keys below) of two wave-uniform keys in the sorted
+// ckey[0, n), n >= 1: two 64-ary searches in lockstep, one probe per lane per
+// key and round, then a last round over at most 64 keys.
+__device__ __forceinline__ void wave_rank2(gptr<const uint64_t> ckey, int n, uint64_t k1, uint64_t k2, int *r1,
+                                           int *r2) {
+    const int lane = lane_id();
+    int lo1 = 0, hi1 = n, lo2 = 0, hi2 = n;  // rank in [lo, hi]
+    while (hi1 - lo1 > 64 || hi2 - lo2 > 64) {
+        const int s1 = hi1 - lo1, s2 = hi2 - lo2;
+        const int p1 = lo1 + (int)((long long)(lane + 1) * s1 / 65);  // in [lo, hi) when s > 64
+        const int p2 = lo2 + (int)((long long)(lane + 1) * s2 / 65);
+        const uint64_t v1 = ckey[min(p1, n - 1)];
+        const uint64_t v2 = ckey[min(p2, n - 1)];
+        if (s1 > 64) {
+            const int c = __popcll(__ballot(v1 < k1));  // the probes below k1 are a prefix of the lanes
+            const int nlo = c > 0 ? __builtin_amdgcn_readlane(p1, c - 1) + 1 : lo1;
+            hi1 = c < 64 ? __builtin_amdgcn_readlane(p1, c) : hi1;
+            lo1 = nlo;
+        }
+        if (s2 > 64) {
+            const int c = __popcll(__ballot(v2 < k2));
+            const int nlo = c > 0 ? __builtin_amdgcn_readlane(p2, c - 1) + 1 : lo2;
+            hi2 = c < 64 ? __builtin_amdgcn_readlane(p2, c) : hi2;
+            lo2 = nlo;
         }
     }
-    int first = -1, lastc = -1;
-#pragma unroll
-    for (int i = 0; i < 2 * kMaxK + 1; i++) {
-        if (i < nz) {
-            // continue the probe sequence where the first probe was inconclusive
-            uint32_t hh = h[i];
-            uint64_t cv = cur[i];
-            while (cv != tag[i] && (uint32_t)(cv >> kKeyBits) == epoch) {
-                hh = (hh + 1) & S.hmask;
-                cv = S.htab[hh];
-            }
-            if (cv == tag[i]) {
-                const int c2 = S.hval[hh];
-                if (first < 0) first = c2;
-                lastc = c2;
-            }
-        }
+    const bool b1 = lo1 + lane < hi1 && ckey[lo1 + lane] < k1;
+    const bool b2 = lo2 + lane < hi2 && ckey[lo2 + lane] < k2;
+    *r1 = lo1 + __popcll(__ballot(b1));
+    *r2 = lo2 + __popcll(__ballot(b2));
+}
+
+// First index in sorted a[0, n) (n >= 1) whose key is >= key.
+template <class P>
+__device__ __forceinline__ int lower_index(P a, int n, uint64_t key) {
+    int base = 0;
+    while (n > 1) {
+        const int half = n >> 1;
+        if (a[base + half] < key) base += half;
+        n -= half;
     }
-    if (first < 0) return make_int2(0, 0);
-    return make_int2(S.cstart[first], S.cstart[lastc + 1]);
+    return base + (a[base] < key ? 1 : 0);
 }
 
 // Cell classes (written once per slide by k_rows):
@@ -793,24 +773,97 @@ __device__ __forceinline__ int2 cell_row(const StreamDev &S, int c, int r, uint3
 //                cells may hold points clamped in from outside the grid
 //                bounds, which the side-length argument does not cover.
 //   kCellDense - safe and holding >= min_pts points: all of them are core.
-__device__ __forceinline__ uint8_t cell_class(const StreamDev &S, int c) {
-    if (!S.dense_ok) return 0;
-    int64_t ix[3];
-    decode_key(S, S.ckey[c], ix);
-    for (int a = 0; a < S.dim; a++)
-        if (ix[a] <= 0 || ix[a] >= S.n[a] - 1) return 0;
-    const int cnt = S.cstart[c + 1] - S.cstart[c];
-    return cnt >= S.min_pts ? (kCellSafe | kCellDense) : kCellSafe;
+__device__ __forceinline__ uint8_t cell_class(const RowsView &V, int64_t i0, int64_t i1, int64_t i2, int c) {
+    if (!V.dense_ok) return 0;
+    if (i0 <= 0 || i0 >= V.n0 - 1) return 0;
+    if (V.dim > 1 && (i1 <= 0 || i1 >= V.n1 - 1)) return 0;
+    if (V.dim > 2 && (i2 <= 0 || i2 >= V.n2 - 1)) return 0;
+    const int cnt = V.cstart[c + 1] - V.cstart[c];
+    return cnt >= V.min_pts ? (kCellSafe | kCellDense) : kCellSafe;
 }
 
-// Persistent: walks the device-side list of (stream, cell, row) items. The
-// row-0 item of each cell also records the cell's class; block 0 publishes
-// the per-stream prefix of cell counts for the cell-parallel kernels.
+// Row r (centre row first) of the 64 cells of tile t, one per lane; the
+// centre-row item also records each cell's class. lk/lc: this wave's LDS.
+__device__ __forceinline__ void rows_tile(const RowsView &V, int t, int r, uint64_t *lk, int32_t *lc) {
+    const int lane = lane_id();
+    const int c = t * kRowTile + lane;
+    const bool cv = c < V.ncell;
+    const uint64_t ck = cv ? V.ckey[c] : 0ull;
+    int64_t q, i0, i1, i2;
+    divmod_key(ck, V.n2, &q, &i2);
+    divmod_key((uint64_t)q, V.n1, &i0, &i1);
+    // row offsets on the other axes (uniform)
+    const int k = V.k, w = 2 * k + 1, center = (V.R - 1) / 2;
+    int rr = r == 0 ? center : (r == center ? 0 : r);
+    int o0 = 0, o1 = 0;
+    bool ok = cv;
+    int64_t zc, zn;
+    if (V.dim == 3) {
+        o1 = rr % w - k;
+        rr /= w;
+        o0 = rr % w - k;
+        ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0 && i1 + o1 >= 0 && i1 + o1 < V.n1;
+        zc = i2;
+        zn = V.n2;
+    } else if (V.dim == 2) {
+        o0 = rr % w - k;
+        ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0;
+        zc = i1;
+        zn = V.n1;
+    } else {
+        zc = i0;
+        zn = V.n0;
+    }
+    // key bounds: the target column's keys at zlo..zhi
+    const int64_t colbase = (int64_t)ck - zc + ((int64_t)o0 * V.n1 + o1) * V.n2;
+    const uint64_t lo = (uint64_t)(colbase + (zc - k < 0 ? 0 : zc - k));
+    const uint64_t hi = (uint64_t)(colbase + (zc + k >= zn ? zn - 1 : zc + k));
+    uint64_t row = 0ull;
+    const uint64_t okm = __ballot(ok);
+    if (okm) {
+        // bounds rise with the lane (see above): the first and last lanes in
+        // the grid give the bracket
+        const uint64_t bmin = readlane64(lo, __ffsll((unsigned long long)okm) - 1);
+        const uint64_t bmax = readlane64(hi, 63 - __clzll((long long)okm));
+        int b0, b1;
+        wave_rank2(V.ckey, V.ncell, bmin, bmax + 1, &b0, &b1);
+        const int B = b1 - b0;
+        if (B > 0 && B <= kRowCap) {
+            for (int i = lane; i < B; i += 64) lk[i] = V.ckey[b0 + i];
+            for (int i = lane; i <= B; i += 64) lc[i] = V.cstart[b0 + i];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (ok) {
+                const int f = lower_index(lk, B, lo), e = lower_index(lk, B, hi + 1);
+                if (f < e) row = (uint64_t)(uint32_t)lc[f] | ((uint64_t)(uint32_t)lc[e] << 32);
+            }
+            __builtin_amdgcn_wave_barrier();  // reads done before the next item's stores
+        } else if (B > 0 && ok) {
+            const int f = b0 + lower_index(V.ckey + b0, B, lo), e = b0 + lower_index(V.ckey + b0, B, hi + 1);
+            if (f < e) row = (uint64_t)(uint32_t)V.cstart[f] | ((uint64_t)(uint32_t)V.cstart[e] << 32);
+        }
+    }
+    if (cv) {
+        V.rows[(size_t)c * V.R + r] = row;
+        if (r == 0) {
+            V.cflag[c] = cell_class(V, i0, i1, i2, c);
+            V.fcore[c] = kNoCore;
+        }
+    }
+}
+
+// Persistent: walks the device-side list of (stream, 64-cell tile, row)
+// items, one per wave. Block 0 also publishes the per-stream prefix of cell
+// counts for the cell kernels.
 __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
     __shared__ long long pre[kMaxBatch + 1];
     __shared__ long long tmp4[4];
-    block_stream_prefix(a.n, pre, tmp4,
-                        [&](int i) { return (long long)a.s[i].mut->ncell * a.s[i].R; });
+    __shared__ uint64_t lkey[kRowWaves][kRowCap];
+    __shared__ int32_t lcs[kRowWaves][kRowCap + 1];
+    block_stream_prefix(a.n, pre, tmp4, [&](int i) {
+        return (long long)((a.s[i].mut->ncell + kRowTile - 1) / kRowTile) * a.s[i].R;
+    });
     if (blockIdx.x == 0) {
         long long run = 0;
         if (threadIdx.x == 0) {
@@ -822,17 +875,13 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
         }
     }
     const long long total = pre[a.n];
-    for (long long gi = (long long)blockIdx.x * kTileThreads + threadIdx.x; gi < total;
-         gi += (long long)gridDim.x * kTileThreads) {
-        const int s = find_stream(pre, a.n, gi);
-        const StreamDev &S = a.s[s];
-        const long long local = gi - pre[s];
-        const int c = (int)(local / S.R), r = (int)(local % S.R);
-        S.rows[(size_t)c * S.R + r] = cell_row(S, c, r, S.mut->epoch);
-        if (r == 0) {
-            S.cflag[c] = cell_class(S, c);
-            S.fcore[c] = kNoCore;
-        }
+    const int wv = uni(threadIdx.x >> 6);
+    for (long long wi = (long long)blockIdx.x * kRowWaves + wv; wi < total; wi += (long long)gridDim.x * kRowWaves) {
+        const int s = uni(find_stream(pre, a.n, wi));
+        const RowsView V = rows_view(a.s[s]);
+        const int local = uni((int)(wi - pre[s]));
+        const int t = local / V.R;
+        rows_tile(V, t, local - t * V.R, lkey[wv], lcs[wv]);
     }
 }
 
@@ -1971,7 +2020,7 @@ __global__ __launch_bounds__(64) void k_union_points(BatchArgs a) { union_points
 
 // Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
 // root of the core's representative. Block 0 of each stream also advances the
-// ring cursor and hash epoch; no later kernel of the slide reads them.
+// ring cursor; no later kernel of the slide reads it.
 __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
     const ChunkCtx x = chunk_of(a, g);
     const StreamDev &S = a.s[x.s];
@@ -1982,7 +2031,6 @@ __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
         if (a.stats) a.stats[(size_t)x.s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
         m->t = m->t + 1;
         m->u = m->u + 1 == S.units ? 0 : m->u + 1;
-        m->epoch = m->epoch >= kEpochMax ? 1u : m->epoch + 1u;
     }
     if (p >= x.p1) return;
     if (!(S.flags[p] & kFlagCore)) return;
@@ -1994,15 +2042,6 @@ __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
 }
 
 __global__ __launch_bounds__(64) void k_flatten(BatchArgs a) { flatten_chunk(a, blockIdx.x); }
-
-__global__ __launch_bounds__(kTileThreads) void k_clear_hash(BatchArgs a) {
-    const StreamDev &S = a.s[blockIdx.y];
-    const size_t sz = (size_t)S.hmask + 1;
-    for (size_t i = (size_t)blockIdx.x * kTileThreads + threadIdx.x; i < sz;
-         i += (size_t)gridDim.x * kTileThreads)
-        S.htab[i] = 0ull;
-    if (blockIdx.x == 0 && threadIdx.x == 0) S.mut->epoch = 1u;
-}
 
 // Plain device copy of 16-byte words: a kernel of known HBM traffic (reads and
 // writes `n16` * 16 bytes) used to scale the profiler's byte counters.
@@ -2109,11 +2148,6 @@ hipError_t launch_init(const BatchArgs &a, int32_t itiles, int32_t passes, hipSt
     RTDBS_LAUNCH(k_init_tiles, itiles, kTileThreads, a);
     for (int p = 0; p < passes; p++) RTDBS_LAUNCH(k_init_merge, itiles, kTileThreads, a, kTile << p, p & 1);
     return hipSuccess;
-}
-
-hipError_t launch_clear_hash(const BatchArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear_hash, dim3(256, a.n), dim3(kTileThreads), 0, st, a);
-    return hipGetLastError();
 }
 
 }  // namespace rtdbs
