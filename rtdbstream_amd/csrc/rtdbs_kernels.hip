@@ -1103,16 +1103,26 @@ struct RowWalk {
     int base = 0;         // position minus candidate index in the last row opened (uniform)
 };
 
-// Rows of cell A, each cut at window position `limit`.
-__device__ __forceinline__ RowWalk row_walk(const CellView &V, int A, int limit) {
-    RowWalk w;
+// Row r of cell A in lane r < R (int2 as one word), 0 in the other lanes.
+__device__ __forceinline__ uint64_t row_word(const CellView &V, int A) {
     const int lane = lane_id();
-    if (lane < V.R) {
-        const uint64_t r = V.rows[(size_t)A * V.R + lane];
-        w.rx = (int)(uint32_t)r;
-        w.len = max(0, min((int)(uint32_t)(r >> 32), limit) - w.rx);
-    }
+    return lane < V.R ? V.rows[(size_t)A * V.R + lane] : 0ull;
+}
+
+// The walk over rows (row_word), each cut at window position `limit`.
+__device__ __forceinline__ RowWalk row_walk(uint64_t rw, int limit) {
+    RowWalk w;
+    w.rx = (int)(uint32_t)rw;
+    w.len = max(0, min((int)(uint32_t)(rw >> 32), limit) - w.rx);
     return w;
+}
+
+// Waits here for v, loaded earlier. The cell kernels issue every load that
+// depends on the same thing together and then wait for all of them at once,
+// so that none is moved past a branch into a round trip of its own.
+template <class T>
+__device__ __forceinline__ void arrived(T v) {
+    asm volatile("" ::"v"(v));
 }
 
 // Opens the rows that start before jb + 64 and returns this lane's candidate
@@ -1140,11 +1150,18 @@ __device__ __forceinline__ int walk_chunk(RowWalk &w, int R, int jb, int *end) {
 // __intersection__grid, dbscan.cu:155-194,250-270, and set_label,
 // aabb.cu:575-587). Dense cells are core without counting (all pairs within
 // eps). Otherwise each query counts its neighbours, itself included, and
-// stops once the count reaches min_pts.
+// stops once the count reaches min_pts. Loads are issued in three rounds -
+// the cell's header and rows, its queries, each candidate chunk - each
+// waited for as a whole.
 __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
     const int lane = lane_id();
-    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
-    const int cf = uni(V.cflag[A]);
+    const int R = V.R, dim = V.dim, minpts = V.min_pts;
+    const float e2 = V.eps2;
+    const int csv = V.cstart[A], cev = V.cstart[A + 1], cfv = V.cflag[A];
+    const uint64_t rw = row_word(V, A);
+    arrived(rw);
+    const int cs = uni(csv), nA = uni(cev) - cs;
+    const int cf = uni(cfv);
     if (cf & kCellDense) {
         st.v[kStatDensePoints] += nA;
         const int rs = uni(vslot(V, cs));
@@ -1157,20 +1174,17 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
         if (lane == 0) V.fcore[A] = cs;
         return;
     }
-    const int R = V.R, dim = V.dim, minpts = V.min_pts;
-    const float e2 = V.eps2;
-    const RowWalk w0 = row_walk(V, A, INT_MAX);
-    int fc = kNoCore;
+    const bool safe = cf & kCellSafe;
+    const RowWalk w0 = row_walk(rw, INT_MAX);
+    int fc = kNoCore, fslot = 0;
     st.v[kStatQueries] += nA;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
-        const int pq = cs + q0 + lane;
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        if (lane < nq) {
-            qx = V.sx[pq];
-            if (dim > 1) qy = V.sy[pq];
-            if (dim > 2) qz = V.sz[pq];
-        }
+        const int pq = cs + q0 + min(lane, nq - 1);
+        const float qx = V.sx[pq];
+        const float qy = dim > 1 ? V.sy[pq] : 0.f;
+        const float qz = dim > 2 ? V.sz[pq] : 0.f;
+        const int qslot = vslot(V, pq);
         int mine = 0;
         uint64_t open = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
         RowWalk w = w0;
@@ -1194,19 +1208,21 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
             open &= ~__ballot(mine >= minpts);
         }
         if (lane < nq) {
-            const int sl = vslot(V, pq);
-            V.parent[sl] = sl;
-            V.flags[pq] = mine >= minpts ? kFlagCore : 0;
+            V.parent[qslot] = qslot;
+            V.flags[cs + q0 + lane] = mine >= minpts ? kFlagCore : 0;
+            if (!safe) V.rep[cs + q0 + lane] = qslot;  // unsafe cell: every core its own element
         }
         const uint64_t cm = __ballot(lane < nq && mine >= minpts);
-        if (fc == kNoCore && cm != 0ull) fc = cs + q0 + __ffsll((unsigned long long)cm) - 1;
+        if (fc == kNoCore && cm != 0ull) {
+            const int l = __ffsll((unsigned long long)cm) - 1;
+            fc = cs + q0 + l;
+            fslot = lane_i(qslot, l);
+        }
     }
     if (lane == 0) V.fcore[A] = fc;  // first (minimum-slot) core: the cell's representative
-    // union-find element of each point: the representative's slot in a safe
-    // cell, else its own slot
-    const bool safe = cf & kCellSafe;
-    const int rs = fc != kNoCore ? uni(vslot(V, fc)) : 0;
-    for (int i = lane; i < nA; i += 64) V.rep[cs + i] = safe ? rs : vslot(V, cs + i);
+    // safe cell: one union-find element, the representative's slot
+    if (safe)
+        for (int i = lane; i < nA; i += 64) V.rep[cs + i] = fc != kNoCore ? fslot : 0;
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
@@ -1224,43 +1240,47 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
 
 // Border / noise (the reference's border claim, dbscan.cu:370-377, with the
 // deterministic rule: the minimum canonical id among cores within eps) for
-// the non-core points of a cell; cells without them are skipped.
+// the non-core points of a cell; cells without them are skipped. Loads in
+// rounds, as in count_cell.
 __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &st) {
     const int lane = lane_id();
-    if (uni(V.cflag[A]) & kCellDense) return;
-    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
     const int R = V.R, dim = V.dim;
     const float e2 = V.eps2;
+    const int cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
+    const uint64_t rw = row_word(V, A);
+    arrived(rw);
+    if (uni(cfv) & kCellDense) return;
+    const int cs = uni(csv), nA = uni(cev) - cs;
     gptr<int32_t> label = gl(io.label), cidv = gl(io.cid);
-    RowWalk w0;
-    bool have_rows = false;
+    const RowWalk w0 = row_walk(rw, INT_MAX);
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
-        const int pq = cs + q0 + lane;
-        const bool isq = lane < nq && !(V.flags[pq] & kFlagCore);
+        const int pq = cs + q0 + min(lane, nq - 1);
+        const int qf = V.flags[pq];
+        const float qx = V.sx[pq];
+        const float qy = dim > 1 ? V.sy[pq] : 0.f;
+        const float qz = dim > 2 ? V.sz[pq] : 0.f;
+        const uint64_t qk = V.skey[pq];
+        arrived(qx);
+        arrived(qy);
+        arrived(qz);
+        arrived(qk);
+        const bool isq = lane < nq && !(qf & kFlagCore);
         const uint64_t open = __ballot(isq);
         if (open == 0ull) continue;
-        if (!have_rows) {
-            w0 = row_walk(V, A, INT_MAX);
-            have_rows = true;
-        }
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        if (isq) {
-            qx = V.sx[pq];
-            if (dim > 1) qy = V.sy[pq];
-            if (dim > 2) qz = V.sz[pq];
-        }
         int best = INT_MAX;
         RowWalk w = w0;
         for (int jb = 0;; jb += 64) {
             int end;
             const int pos = walk_chunk(w, R, jb, &end);
             if (jb >= end) break;
+            const int f = V.flags[pos];
             const float cx = V.sx[pos];
             const float cy = dim > 1 ? V.sy[pos] : 0.f;
             const float cz = dim > 2 ? V.sz[pos] : 0.f;
-            const bool core = jb + lane < end && (V.flags[pos] & kFlagCore);
-            const int cid = core ? V.canon[pos] : INT_MAX;
+            const int id = V.canon[pos];  // meaningful for cores only
+            const bool core = jb + lane < end && (f & kFlagCore);
+            const int cid = core ? id : INT_MAX;
             uint64_t m = open;
             st.v[kStatBorderTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb);
             while (m) {
@@ -1273,7 +1293,7 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &s
             }
         }
         if (isq) {
-            const int slot = vslot(V, pq);
+            const int slot = (int)((uint32_t)qk & V.smask);
             label[slot] = best != INT_MAX ? 1 : 2;
             cidv[slot] = best != INT_MAX ? best : slot;
         }
@@ -1342,42 +1362,55 @@ __device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int r
 // candidate's element is its safe cell's representative or its own slot.
 __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
     const int lane = lane_id();
-    const int fcA = uni(V.fcore[A]);
-    if (fcA == kNoCore) return;
-    const int cfA = uni(V.cflag[A]);
-    const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
-    const int cs = uni(V.cstart[A]), nA = uni(V.cstart[A + 1]) - cs;
     const int R = V.R, dim = V.dim;
     const float e2 = V.eps2;
-    const RowWalk w0 = row_walk(V, A, safeA ? cs : cs + nA);
+    // loads in rounds, as in count_cell: header and rows, queries, chunks
+    const int fcv = V.fcore[A], cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
+    const uint64_t rw = row_word(V, A);
+    arrived(rw);
+    const int fcA = uni(fcv);
+    if (fcA == kNoCore) return;
+    const int cfA = uni(cfv);
+    const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
+    const int cs = uni(csv), nA = uni(cev) - cs;
+    const RowWalk w0 = row_walk(rw, safeA ? cs : cs + nA);
     if (__ballot(w0.len > 0) == 0ull) return;
-    const int repA = uni(vslot(V, fcA));
+    const uint64_t rk = V.skey[fcA];  // the representative's slot, with the first queries
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
-        const int pq = cs + q0 + lane;
-        const bool qcore = lane < nq && (denseA || (V.flags[pq] & kFlagCore));
+        const int pq = cs + q0 + min(lane, nq - 1);
+        const int qf = V.flags[pq];
+        const float qx = V.sx[pq];
+        const float qy = dim > 1 ? V.sy[pq] : 0.f;
+        const float qz = dim > 2 ? V.sz[pq] : 0.f;
+        const uint64_t qk = V.skey[pq];
+        arrived(qx);
+        arrived(qy);
+        arrived(qz);
+        arrived(qk);
+        const bool qcore = lane < nq && (denseA || (qf & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        int qslot = 0;
-        if (qcore) {
-            qx = V.sx[pq];
-            if (dim > 1) qy = V.sy[pq];
-            if (dim > 2) qz = V.sz[pq];
-            qslot = vslot(V, pq);
-        }
+        const int qslot = (int)((uint32_t)qk & V.smask);
+        const int repA = uni((int)((uint32_t)rk & V.smask));
         RowWalk w = w0;
         for (int jb = 0;; jb += 64) {
             int end;
             const int pos = walk_chunk(w, R, jb, &end);
             if (jb >= end) break;
-            const bool bcore = jb + lane < end && (V.flags[pos] & kFlagCore);
-            const uint64_t bm = __ballot(bcore);
-            if (bm == 0ull) continue;
+            const int f = V.flags[pos];
             const float cx = V.sx[pos];
             const float cy = dim > 1 ? V.sy[pos] : 0.f;
             const float cz = dim > 2 ? V.sz[pos] : 0.f;
-            const int rep = bcore ? V.rep[pos] : 0;
+            const int rp = V.rep[pos];  // meaningful for cores only
+            arrived(cx);
+            arrived(cy);
+            arrived(cz);
+            arrived(rp);
+            const bool bcore = jb + lane < end && (f & kFlagCore);
+            const uint64_t bm = __ballot(bcore);
+            if (bm == 0ull) continue;
+            const int rep = bcore ? rp : 0;
             const unsigned chunk = (unsigned)min(64, end - jb);
             uint64_t m = qm;
             if (safeA) {
