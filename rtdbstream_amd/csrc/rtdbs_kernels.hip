@@ -1094,12 +1094,13 @@ __device__ __forceinline__ int vslot(const CellView &V, int pos) { return (int)(
 
 // A cell's stencil candidates, 64 at a time. Lane r < R holds stencil row r:
 // its first window position and its length. A walk in scalar registers opens
-// the rows in order (reading each with readlane) and gives every lane the
-// position of its candidate, which belongs to the last row opened at or
-// before it - no cross-lane search per chunk.
+// the non-empty rows in order (a ballot mask; each row read with readlane)
+// and gives every lane the position of its candidate, which belongs to the
+// last row opened at or before it - no cross-lane search per chunk.
 struct RowWalk {
     int rx = 0, len = 0;  // lane r: row r
-    int r = 0, acc = 0;   // next row to open and its first candidate (uniform)
+    uint64_t rest = 0;    // non-empty rows not opened yet (uniform)
+    int acc = 0;          // first candidate of the next row to open (uniform)
     int base = 0;         // position minus candidate index in the last row opened (uniform)
 };
 
@@ -1114,6 +1115,7 @@ __device__ __forceinline__ RowWalk row_walk(uint64_t rw, int limit) {
     RowWalk w;
     w.rx = (int)(uint32_t)rw;
     w.len = max(0, min((int)(uint32_t)(rw >> 32), limit) - w.rx);
+    w.rest = __ballot(w.len > 0);
     return w;
 }
 
@@ -1129,20 +1131,18 @@ __device__ __forceinline__ void arrived(T v) {
 // position in chunk [jb, jb + 64); lanes past the last candidate get the last
 // one. *end is the candidate count once every row is open, else INT_MAX.
 // Callers stop at jb >= *end before using the position.
-__device__ __forceinline__ int walk_chunk(RowWalk &w, int R, int jb, int *end) {
+__device__ __forceinline__ int walk_chunk(RowWalk &w, int jb, int *end) {
     const int j = jb + lane_id();
     int base = w.base;
-    while (w.r < R && w.acc < jb + 64) {
-        const int len = __builtin_amdgcn_readlane(w.len, w.r);
-        if (len > 0) {
-            const int b = __builtin_amdgcn_readlane(w.rx, w.r) - w.acc;
-            if (j >= w.acc) base = b;
-            w.base = b;
-            w.acc += len;
-        }
-        w.r++;
+    while (w.rest != 0ull && w.acc < jb + 64) {
+        const int r = __ffsll((unsigned long long)w.rest) - 1;
+        w.rest &= w.rest - 1ull;
+        const int b = __builtin_amdgcn_readlane(w.rx, r) - w.acc;
+        if (j >= w.acc) base = b;
+        w.base = b;
+        w.acc += __builtin_amdgcn_readlane(w.len, r);
     }
-    *end = w.r < R ? INT_MAX : w.acc;
+    *end = w.rest != 0ull ? INT_MAX : w.acc;
     return base + min(j, *end - 1);
 }
 
@@ -1155,7 +1155,7 @@ __device__ __forceinline__ int walk_chunk(RowWalk &w, int R, int jb, int *end) {
 // waited for as a whole.
 __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
     const int lane = lane_id();
-    const int R = V.R, dim = V.dim, minpts = V.min_pts;
+    const int dim = V.dim, minpts = V.min_pts;
     const float e2 = V.eps2;
     const int csv = V.cstart[A], cev = V.cstart[A + 1], cfv = V.cflag[A];
     const uint64_t rw = row_word(V, A);
@@ -1190,7 +1190,7 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
         RowWalk w = w0;
         for (int jb = 0; open; jb += 64) {
             int end;
-            const int pos = walk_chunk(w, R, jb, &end);
+            const int pos = walk_chunk(w, jb, &end);
             if (jb >= end) break;
             const float cx = V.sx[pos];
             const float cy = dim > 1 ? V.sy[pos] : 0.f;
@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
 // rounds, as in count_cell.
 __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &st) {
     const int lane = lane_id();
-    const int R = V.R, dim = V.dim;
+    const int dim = V.dim;
     const float e2 = V.eps2;
     const int cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
     const uint64_t rw = row_word(V, A);
@@ -1272,7 +1272,7 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &s
         RowWalk w = w0;
         for (int jb = 0;; jb += 64) {
             int end;
-            const int pos = walk_chunk(w, R, jb, &end);
+            const int pos = walk_chunk(w, jb, &end);
             if (jb >= end) break;
             const int f = V.flags[pos];
             const float cx = V.sx[pos];
@@ -1362,7 +1362,7 @@ __device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int r
 // candidate's element is its safe cell's representative or its own slot.
 __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
     const int lane = lane_id();
-    const int R = V.R, dim = V.dim;
+    const int dim = V.dim;
     const float e2 = V.eps2;
     // loads in rounds, as in count_cell: header and rows, queries, chunks
     const int fcv = V.fcore[A], cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
@@ -1374,7 +1374,7 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
     const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
     const int cs = uni(csv), nA = uni(cev) - cs;
     const RowWalk w0 = row_walk(rw, safeA ? cs : cs + nA);
-    if (__ballot(w0.len > 0) == 0ull) return;
+    if (w0.rest == 0ull) return;
     const uint64_t rk = V.skey[fcA];  // the representative's slot, with the first queries
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
@@ -1396,7 +1396,7 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
         RowWalk w = w0;
         for (int jb = 0;; jb += 64) {
             int end;
-            const int pos = walk_chunk(w, R, jb, &end);
+            const int pos = walk_chunk(w, jb, &end);
             if (jb >= end) break;
             const int f = V.flags[pos];
             const float cx = V.sx[pos];
