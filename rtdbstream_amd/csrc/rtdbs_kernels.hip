@@ -35,13 +35,26 @@ const char *const kPhaseNames[kNumPhases] = {"sort_expire", "rank_compact", "mer
 // small helpers
 
 
-// Largest s in [0, n) with pre[s] <= g (pre[0] = 0, g < pre[n]).
+// Largest s in [0, n) with pre[s] <= g (pre[0] = 0, g < pre[n]); pre in LDS.
 template <class T>
 __device__ __forceinline__ int find_stream(const T *pre, int n, long long g) {
     int lo = 0, hi = n;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if ((long long)pre[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The same over a per-batch prefix table in global memory, which no kernel
+// writes during a launch: scalar loads (the table stays in the scalar cache),
+// not a dependent vector load per step.
+__device__ __forceinline__ int find_stream_c(const int32_t *pre, int n, long long g) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if ((long long)ldc(pre[mid]) <= g) lo = mid;
         else hi = mid;
     }
     return lo;
@@ -244,10 +257,10 @@ __device__ __forceinline__ void sort_tile_keys(const StreamDev &S, const float *
 __global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
     __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
-    const int s = find_stream(a.pre_itile, a.n, g);
+    const int s = find_stream_c(a.pre_itile, a.n, g);
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int tile0 = (g - a.pre_itile[s]) * kTile;
+    const int tile0 = (g - ldc(a.pre_itile[s])) * kTile;
     uint64_t v[8];
     sort_tile_keys<kTile>(S, io.in, S.W, tile0, 0, v, lds, threadIdx.x);
     const int nvalid = min(kTile, S.W - tile0);
@@ -275,9 +288,9 @@ __device__ __forceinline__ bool survives(const StreamDev &S, uint64_t key, int l
 // Survivors per window tile (the reference's stable_partition,
 // thrust_helper.cu:37-39, split into count + compact).
 __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds4) {
-    const int s = find_stream(a.pre_wtile, a.n, g);
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
+    const int tile = g - ldc(a.pre_wtile[s]);
     const int p0 = tile * kWTile, p1 = min(p0 + kWTile, S.W);
     const int lo_slot = S.mut->u * S.S;
     uint64_t k[kWTile / kTileThreads];
@@ -303,15 +316,15 @@ constexpr int kRun = 512;  // keys per sorted run of the stride (one wave)
 __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
-    const int nst = a.pre_stile[a.n];
+    const int nst = ldc(a.pre_stile[a.n]);
     if (g >= nst) {
         expire_count(a, g - nst, (int *)lds);
         return;
     }
-    const int s = find_stream(a.pre_stile, a.n, g);
+    const int s = find_stream_c(a.pre_stile, a.n, g);
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int tile0 = (g - a.pre_stile[s]) * kTile;
+    const int tile0 = (g - ldc(a.pre_stile[s])) * kTile;
     const int w = threadIdx.x >> 6;
     uint64_t v[8];
     sort_tile_keys<kRun>(S, io.in, S.S, tile0 + w * kRun, S.mut->u * S.S, v, nullptr, lane_id());
@@ -326,9 +339,9 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
 // Ordered compaction of the survivors into the A arrays. Each tile's offset
 // is the sum of the preceding tiles' counts (no spin-waiting look-back).
 __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *lds4) {
-    const int s = find_stream(a.pre_wtile, a.n, g);
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
+    const int tile = g - ldc(a.pre_wtile[s]);
     const int p0 = tile * kWTile;
     const int lo_slot = S.mut->u * S.S;
     constexpr int per = kWTile / kTileThreads;
@@ -378,16 +391,16 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
     __shared__ uint64_t lds[kStageRuns * kRun];
     const int g = blockIdx.x;
     const int gb = g / kRankParts, part = g % kRankParts;
-    const int s = find_stream(a.pre_stile, a.n, gb);
+    const int s = find_stream_c(a.pre_stile, a.n, gb);
     const StreamDev &S = a.s[s];
-    const int i = (gb - a.pre_stile[s]) * kTile + part * kTileThreads + threadIdx.x;
+    const int i = (gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads + threadIdx.x;
     const bool valid = i < S.S;
     const uint64_t key = valid ? S.bkey0[i] : ~0ull;
     const int myrun = valid ? i / kRun : -1, nr = (S.S + kRun - 1) / kRun;
     int rank = valid ? i - myrun * kRun : 0;
     // the runs other than this block's own ones are staged; a block's keys
     // share at most one run (kTileThreads divides kRun)
-    const int blockrun = ((gb - a.pre_stile[s]) * kTile + part * kTileThreads) / kRun;
+    const int blockrun = ((gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads) / kRun;
     for (int o0 = 0; o0 < nr - 1; o0 += kStageRuns) {
         int len[kStageRuns];
         constexpr int per = kStageRuns * kRun / kTileThreads;
@@ -535,10 +548,10 @@ __device__ __forceinline__ void merge_tile(const uint64_t *A, int la, const uint
 __global__ __launch_bounds__(kTileThreads) void k_init_merge(BatchArgs a, int L, int sel) {
     __shared__ MergeLds<kTile> m;
     const int g = blockIdx.x;
-    const int s = find_stream(a.pre_itile, a.n, g);
+    const int s = find_stream_c(a.pre_itile, a.n, g);
     const StreamDev &S = a.s[s];
     const int len = S.W;
-    const int o0 = (g - a.pre_itile[s]) * kTile;
+    const int o0 = (g - ldc(a.pre_itile[s])) * kTile;
     const int o1 = min(o0 + kTile, len);
     const uint64_t *sk = sel == 0 ? S.akey : S.skey;
     const float *sx = sel == 0 ? S.ax : S.sx, *sy = sel == 0 ? S.ay : S.sy, *sz = sel == 0 ? S.az : S.sz;
@@ -571,9 +584,9 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
     __shared__ MergeLds<kWTile> m;
     __shared__ int lds4[4];
     const int g = blockIdx.x;
-    const int s = find_stream(a.pre_wtile, a.n, g);
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
+    const int tile = g - ldc(a.pre_wtile[s]);
     const int o0 = tile * kWTile, o1 = min(o0 + kWTile, S.W);
     const int la = S.W - S.S, lb = S.S;
     const uint64_t *B = S.bkey1;
@@ -625,10 +638,10 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
 __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     __shared__ int lds4[4];
     const int g = blockIdx.x;
-    const int s = find_stream(a.pre_wtile, a.n, g);
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int tile = g - a.pre_wtile[s];
-    const int ntile = a.pre_wtile[s + 1] - a.pre_wtile[s];
+    const int tile = g - ldc(a.pre_wtile[s]);
+    const int ntile = ldc(a.pre_wtile[s + 1]) - ldc(a.pre_wtile[s]);
     const int p0 = tile * kWTile;
     const int sb = S.slot_bits;
     constexpr int per = kWTile / kTileThreads;
@@ -912,8 +925,8 @@ struct CellCtx {
 
 __device__ __forceinline__ CellCtx cell_of(const BatchArgs &a, int gc) {
     CellCtx x;
-    x.s = find_stream(a.cell_pre, a.n, gc);
-    x.c = gc - a.cell_pre[x.s];
+    x.s = find_stream_c(a.cell_pre, a.n, gc);
+    x.c = gc - ldc(a.cell_pre[x.s]);
     return x;
 }
 
@@ -924,9 +937,9 @@ struct ChunkCtx {
 
 __device__ __forceinline__ ChunkCtx chunk_of(const BatchArgs &a, int g) {
     ChunkCtx x;
-    x.s = find_stream(a.pre_chunk, a.n, g);
+    x.s = find_stream_c(a.pre_chunk, a.n, g);
     const int W = a.s[x.s].W;
-    x.p0 = (g - a.pre_chunk[x.s]) * kChunk;
+    x.p0 = (g - ldc(a.pre_chunk[x.s])) * kChunk;
     x.p1 = min(x.p0 + kChunk, W);
     return x;
 }
@@ -1226,7 +1239,7 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
-    const int total = uni(a.cell_pre[a.n]);
+    const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
@@ -1301,7 +1314,7 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &s
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
-    const int total = uni(a.cell_pre[a.n]);
+    const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
@@ -1451,7 +1464,7 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     __shared__ int2 edges[kCellWaves][kEdgeCap];
-    const int total = uni(a.cell_pre[a.n]);
+    const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0, nullptr};
@@ -1579,7 +1592,7 @@ __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
 // First (minimum-slot) core of every cell - its union-find representative -
 // one thread per cell.
 __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
-    const int total = a.cell_pre[a.n];
+    const int total = ldc(a.cell_pre[a.n]);
     for (int gc = blockIdx.x * kTileThreads + threadIdx.x; gc < total; gc += gridDim.x * kTileThreads) {
         const CellCtx x = cell_of(a, gc);
         const StreamDev &S = a.s[x.s];
@@ -1604,9 +1617,9 @@ __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
 // representative's slot in a safe cell, else the position's own slot.
 __global__ __launch_bounds__(kTileThreads) void k_rep(BatchArgs a) {
     const int g = blockIdx.x;
-    const int s = find_stream(a.pre_wtile, a.n, g);
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
-    const int p0 = (g - a.pre_wtile[s]) * kWTile;
+    const int p0 = (g - ldc(a.pre_wtile[s])) * kWTile;
 #pragma unroll
     for (int r = 0; r < kWTile / kTileThreads; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
@@ -1847,7 +1860,7 @@ __global__ __launch_bounds__(64) void k_union_cells(BatchArgs a) {
     __shared__ UnionLds L;
     const int lane = lane_id(), grp = lane / kUG, sub = lane % kUG;
     const uint64_t gmask = ((1ull << kUG) - 1ull) << (grp * kUG);
-    const int total = a.cell_pre[a.n];
+    const int total = ldc(a.cell_pre[a.n]);
     // Round r of wave b serves cells b + (r * kUGCells + g) * gridDim.x: the
     // cells of a round are far apart, so dense regions (long pair tests) are
     // spread over many waves instead of falling to a few.
@@ -2059,7 +2072,7 @@ __device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
     const StreamDev &S = a.s[x.s];
     const IO io = a.use_io1 ? a.io1 : a.io[x.s];
     const int p = x.p0 + lane_id();
-    if (g == a.pre_chunk[x.s] && lane_id() == 0) {
+    if (g == ldc(a.pre_chunk[x.s]) && lane_id() == 0) {
         StreamMut *m = S.mut;
         if (a.stats) a.stats[(size_t)x.s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
         m->t = m->t + 1;
