@@ -795,9 +795,15 @@ __device__ __forceinline__ uint8_t cell_class(const RowsView &V, int64_t i0, int
     return cnt >= V.min_pts ? (kCellSafe | kCellDense) : kCellSafe;
 }
 
-// Row r (centre row first) of the 64 cells of tile t, one per lane; the
-// centre-row item also records each cell's class. lk/lc: this wave's LDS.
-__device__ __forceinline__ void rows_tile(const RowsView &V, int t, int r, uint64_t *lk, int32_t *lc) {
+// Rows per item: in 3-D one item holds the 2k+1 rows of one offset o0 on the
+// first axis (o1 = -k..k on the second), whose target columns are adjacent,
+// so one bracket serves all of them; in 2-D and 1-D an item is one row.
+__device__ __forceinline__ int rows_groups(int dim, int k, int R) { return dim == 3 ? 2 * k + 1 : R; }
+
+// Item g of tile t: the rows with offset o0 = g - k (3-D), or row g; the
+// 64 cells of the tile one per lane. The item holding the centre row also
+// records each cell's class. lk/lc: this wave's LDS.
+__device__ __forceinline__ void rows_tile(const RowsView &V, int t, int g, uint64_t *lk, int32_t *lc) {
     const int lane = lane_id();
     const int c = t * kRowTile + lane;
     const bool cv = c < V.ncell;
@@ -805,68 +811,80 @@ __device__ __forceinline__ void rows_tile(const RowsView &V, int t, int r, uint6
     int64_t q, i0, i1, i2;
     divmod_key(ck, V.n2, &q, &i2);
     divmod_key((uint64_t)q, V.n1, &i0, &i1);
-    // row offsets on the other axes (uniform)
     const int k = V.k, w = 2 * k + 1, center = (V.R - 1) / 2;
-    int rr = r == 0 ? center : (r == center ? 0 : r);
-    int o0 = 0, o1 = 0;
-    bool ok = cv;
+    const int nsub = V.dim == 3 ? w : 1;
+    // offsets of the item's rows (uniform): o0 fixed, o1 over [o1lo, o1lo + nsub)
+    int o0 = 0, o1lo = 0;
+    if (V.dim == 3) {
+        o0 = g - k;
+        o1lo = -k;
+    } else if (V.dim == 2) {
+        o0 = g - k;
+    }
     int64_t zc, zn;
     if (V.dim == 3) {
-        o1 = rr % w - k;
-        rr /= w;
-        o0 = rr % w - k;
-        ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0 && i1 + o1 >= 0 && i1 + o1 < V.n1;
         zc = i2;
         zn = V.n2;
     } else if (V.dim == 2) {
-        o0 = rr % w - k;
-        ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0;
         zc = i1;
         zn = V.n1;
     } else {
         zc = i0;
         zn = V.n0;
     }
-    // key bounds: the target column's keys at zlo..zhi
-    const int64_t colbase = (int64_t)ck - zc + ((int64_t)o0 * V.n1 + o1) * V.n2;
-    const uint64_t lo = (uint64_t)(colbase + (zc - k < 0 ? 0 : zc - k));
-    const uint64_t hi = (uint64_t)(colbase + (zc + k >= zn ? zn - 1 : zc + k));
-    uint64_t row = 0ull;
-    const uint64_t okm = __ballot(ok);
-    if (okm) {
-        // bounds rise with the lane (see above): the first and last lanes in
-        // the grid give the bracket
-        const uint64_t bmin = readlane64(lo, __ffsll((unsigned long long)okm) - 1);
-        const uint64_t bmax = readlane64(hi, 63 - __clzll((long long)okm));
-        int b0, b1;
-        wave_rank2(V.ckey, V.ncell, bmin, bmax + 1, &b0, &b1);
-        const int B = b1 - b0;
-        if (B > 0 && B <= kRowCap) {
-            for (int i = lane; i < B; i += 64) lk[i] = V.ckey[b0 + i];
-            for (int i = lane; i <= B; i += 64) lc[i] = V.cstart[b0 + i];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (ok) {
+    // a bracket of the cell table holding every row bound of the item: the
+    // bounds rise with the lane and with o1, and lo >= key + shift - k,
+    // hi <= key + shift + k
+    const uint64_t cvm = __ballot(cv);  // non-empty: t < ceil(ncell / 64)
+    const int64_t klo = (int64_t)readlane64(ck, __ffsll((unsigned long long)cvm) - 1);
+    const int64_t khi = (int64_t)readlane64(ck, 63 - __clzll((long long)cvm));
+    const int64_t sh_lo = ((int64_t)o0 * V.n1 + o1lo) * V.n2, sh_hi = ((int64_t)o0 * V.n1 + o1lo + nsub - 1) * V.n2;
+    const int64_t bmin = klo + sh_lo - k, bmax = khi + sh_hi + k;
+    int b0 = 0, b1 = 0;
+    if (bmax >= 0) wave_rank2(V.ckey, V.ncell, (uint64_t)max(bmin, (int64_t)0), (uint64_t)bmax + 1, &b0, &b1);
+    const int B = b1 - b0;
+    const bool staged = B > 0 && B <= kRowCap;
+    if (staged) {
+        for (int i = lane; i < B; i += 64) lk[i] = V.ckey[b0 + i];
+        for (int i = lane; i <= B; i += 64) lc[i] = V.cstart[b0 + i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for (int j = 0; j < nsub; j++) {
+        const int o1 = o1lo + j;
+        bool ok = cv;
+        if (V.dim == 3) ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0 && i1 + o1 >= 0 && i1 + o1 < V.n1;
+        else if (V.dim == 2) ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0;
+        // key bounds: the target column's keys at zlo..zhi
+        const int64_t colbase = (int64_t)ck - zc + ((int64_t)o0 * V.n1 + (V.dim == 3 ? o1 : 0)) * V.n2;
+        const uint64_t lo = (uint64_t)(colbase + (zc - k < 0 ? 0 : zc - k));
+        const uint64_t hi = (uint64_t)(colbase + (zc + k >= zn ? zn - 1 : zc + k));
+        uint64_t row = 0ull;
+        if (ok && B > 0) {
+            if (staged) {
                 const int f = lower_index(lk, B, lo), e = lower_index(lk, B, hi + 1);
                 if (f < e) row = (uint64_t)(uint32_t)lc[f] | ((uint64_t)(uint32_t)lc[e] << 32);
+            } else {
+                const int f = b0 + lower_index(V.ckey + b0, B, lo), e = b0 + lower_index(V.ckey + b0, B, hi + 1);
+                if (f < e) row = (uint64_t)(uint32_t)V.cstart[f] | ((uint64_t)(uint32_t)V.cstart[e] << 32);
             }
-            __builtin_amdgcn_wave_barrier();  // reads done before the next item's stores
-        } else if (B > 0 && ok) {
-            const int f = b0 + lower_index(V.ckey + b0, B, lo), e = b0 + lower_index(V.ckey + b0, B, hi + 1);
-            if (f < e) row = (uint64_t)(uint32_t)V.cstart[f] | ((uint64_t)(uint32_t)V.cstart[e] << 32);
+        }
+        // stencil row number, centre row first
+        const int rr = V.dim == 3 ? g * w + j : g;
+        const int r = rr == center ? 0 : (rr == 0 ? center : rr);
+        if (cv) {
+            V.rows[(size_t)c * V.R + r] = row;
+            if (r == 0) {
+                V.cflag[c] = cell_class(V, i0, i1, i2, c);
+                V.fcore[c] = kNoCore;
+            }
         }
     }
-    if (cv) {
-        V.rows[(size_t)c * V.R + r] = row;
-        if (r == 0) {
-            V.cflag[c] = cell_class(V, i0, i1, i2, c);
-            V.fcore[c] = kNoCore;
-        }
-    }
+    if (staged) __builtin_amdgcn_wave_barrier();  // reads done before the next item's stores
 }
 
-// Persistent: walks the device-side list of (stream, 64-cell tile, row)
+// Persistent: walks the device-side list of (stream, 64-cell tile, row group)
 // items, one per wave. Block 0 also publishes the per-stream prefix of cell
 // counts for the cell kernels.
 __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
@@ -875,7 +893,8 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
     __shared__ uint64_t lkey[kRowWaves][kRowCap];
     __shared__ int32_t lcs[kRowWaves][kRowCap + 1];
     block_stream_prefix(a.n, pre, tmp4, [&](int i) {
-        return (long long)((a.s[i].mut->ncell + kRowTile - 1) / kRowTile) * a.s[i].R;
+        const StreamDev &S = a.s[i];
+        return (long long)((S.mut->ncell + kRowTile - 1) / kRowTile) * rows_groups(S.dim, S.k, S.R);
     });
     if (blockIdx.x == 0) {
         long long run = 0;
@@ -893,8 +912,9 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
         const int s = uni(find_stream(pre, a.n, wi));
         const RowsView V = rows_view(a.s[s]);
         const int local = uni((int)(wi - pre[s]));
-        const int t = local / V.R;
-        rows_tile(V, t, local - t * V.R, lkey[wv], lcs[wv]);
+        const int G = rows_groups(V.dim, V.k, V.R);
+        const int t = local / G;
+        rows_tile(V, t, local - t * G, lkey[wv], lcs[wv]);
     }
 }
 
