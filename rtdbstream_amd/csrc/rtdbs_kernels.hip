@@ -973,6 +973,49 @@ __device__ __forceinline__ int rep_slot(const StreamDev &S, int j) {
     return slot_of(S, (S.cflag[cj] & kCellSafe) ? S.fcore[cj] : j);
 }
 
+// Wave-uniform view of one stream for the cell-parallel kernels. The
+// descriptor is read with scalar loads (no kernel writes it during a launch)
+// and the arrays are addressed as global memory, so the per-lane accesses
+// are global loads and stores off a scalar base.
+struct CellView {
+    gptr<const float> sx, sy, sz;
+    gptr<const uint64_t> skey;
+    gptr<const int32_t> cstart;
+    gptr<const uint64_t> rows;  // int2 (first, end) per stencil row, as one word
+    gptr<const uint8_t> cflag;
+    gptr<const int32_t> pcell;
+    gptr<uint8_t> flags;
+    gptr<int32_t> fcore, parent, canon, rep;
+    int R, dim, min_pts;
+    uint32_t smask;
+    float eps2;
+};
+
+__device__ __forceinline__ CellView cell_view(const StreamDev &S) {
+    CellView v;
+    v.sx = gl(ldc(S.sx));
+    v.sy = gl(ldc(S.sy));
+    v.sz = gl(ldc(S.sz));
+    v.skey = gl(ldc(S.skey));
+    v.cstart = gl(ldc(S.cstart));
+    v.rows = gl((const uint64_t *)ldc(S.rows));
+    v.cflag = gl(ldc(S.cflag));
+    v.pcell = gl(ldc(S.pcell));
+    v.flags = gl(ldc(S.flags));
+    v.fcore = gl(ldc(S.fcore));
+    v.parent = gl(ldc(S.parent));
+    v.canon = gl(ldc(S.canon));
+    v.rep = gl(ldc(S.rep));
+    v.R = ldc(S.R);
+    v.dim = ldc(S.dim);
+    v.min_pts = ldc(S.min_pts);
+    v.smask = (uint32_t)((1ull << ldc(S.slot_bits)) - 1ull);
+    v.eps2 = ldc(S.eps2);
+    return v;
+}
+
+__device__ __forceinline__ int vslot(const CellView &V, int pos) { return (int)((uint32_t)V.skey[pos] & V.smask); }
+
 constexpr int kRowGroup = 5;  // stencil rows fetched together
 constexpr int kUnroll = 4;    // candidates fetched together
 
@@ -984,31 +1027,31 @@ constexpr int kUnroll = 4;    // candidates fetched together
 // other rows go to sibling threads of the same query).
 // Candidates at positions >= limit are skipped (rows are position ranges).
 template <class Visit>
-__device__ __forceinline__ void walk_candidates(const StreamDev &S, int c, Visit visit, int first = 0,
+__device__ __forceinline__ void walk_candidates(const CellView &V, int c, Visit visit, int first = 0,
                                                 int step = 1, int limit = 0x7fffffff) {
-    const int dim = S.dim, R = S.R;
-    const int2 *rp = S.rows + (size_t)c * R;
+    const int dim = V.dim, R = V.R;
     for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
-        int2 rr[kRowGroup];
+        int rx[kRowGroup], ry[kRowGroup];
 #pragma unroll
         for (int i = 0; i < kRowGroup; i++) {
-            rr[i] = (r0 + i * step < R) ? rp[r0 + i * step] : make_int2(0, 0);
-            rr[i].y = min(rr[i].y, limit);
+            const uint64_t rw = (r0 + i * step < R) ? V.rows[(size_t)c * R + r0 + i * step] : 0ull;
+            rx[i] = (int)(uint32_t)rw;
+            ry[i] = min((int)(uint32_t)(rw >> 32), limit);
         }
 #pragma unroll
         for (int i = 0; i < kRowGroup; i++) {
-            for (int j = rr[i].x; j < rr[i].y; j += kUnroll) {
+            for (int j = rx[i]; j < ry[i]; j += kUnroll) {
                 float cx[kUnroll], cy[kUnroll], cz[kUnroll];
 #pragma unroll
                 for (int u = 0; u < kUnroll; u++) {
-                    const int jj = min(j + u, rr[i].y - 1);
-                    cx[u] = S.sx[jj];
-                    cy[u] = dim > 1 ? S.sy[jj] : 0.f;
-                    cz[u] = dim > 2 ? S.sz[jj] : 0.f;
+                    const int jj = min(j + u, ry[i] - 1);
+                    cx[u] = V.sx[jj];
+                    cy[u] = dim > 1 ? V.sy[jj] : 0.f;
+                    cz[u] = dim > 2 ? V.sz[jj] : 0.f;
                 }
 #pragma unroll
                 for (int u = 0; u < kUnroll; u++)
-                    if (j + u < rr[i].y && visit(j + u, cx[u], cy[u], cz[u])) return;
+                    if (j + u < ry[i] && visit(j + u, cx[u], cy[u], cz[u])) return;
             }
         }
     }
@@ -1083,47 +1126,6 @@ __device__ __forceinline__ void stats_stream(const BatchArgs &a, WaveStats &w, i
         w.s = s;
     }
 }
-
-// Wave-uniform view of one stream for the cell-parallel kernels. The
-// descriptor is read with scalar loads (no kernel writes it during a launch)
-// and the arrays are addressed as global memory, so the per-lane accesses
-// are global loads and stores off a scalar base.
-struct CellView {
-    gptr<const float> sx, sy, sz;
-    gptr<const uint64_t> skey;
-    gptr<const int32_t> cstart;
-    gptr<const uint64_t> rows;  // int2 (first, end) per stencil row, as one word
-    gptr<const uint8_t> cflag;
-    gptr<uint8_t> flags;
-    gptr<int32_t> fcore, parent, canon, rep;
-    int R, dim, min_pts;
-    uint32_t smask;
-    float eps2;
-};
-
-__device__ __forceinline__ CellView cell_view(const StreamDev &S) {
-    CellView v;
-    v.sx = gl(ldc(S.sx));
-    v.sy = gl(ldc(S.sy));
-    v.sz = gl(ldc(S.sz));
-    v.skey = gl(ldc(S.skey));
-    v.cstart = gl(ldc(S.cstart));
-    v.rows = gl((const uint64_t *)ldc(S.rows));
-    v.cflag = gl(ldc(S.cflag));
-    v.flags = gl(ldc(S.flags));
-    v.fcore = gl(ldc(S.fcore));
-    v.parent = gl(ldc(S.parent));
-    v.canon = gl(ldc(S.canon));
-    v.rep = gl(ldc(S.rep));
-    v.R = ldc(S.R);
-    v.dim = ldc(S.dim);
-    v.min_pts = ldc(S.min_pts);
-    v.smask = (uint32_t)((1ull << ldc(S.slot_bits)) - 1ull);
-    v.eps2 = ldc(S.eps2);
-    return v;
-}
-
-__device__ __forceinline__ int vslot(const CellView &V, int pos) { return (int)((uint32_t)V.skey[pos] & V.smask); }
 
 // A cell's stencil candidates, 64 at a time. Lane r < R holds stencil row r:
 // its first window position and its length. A walk in scalar registers opens
@@ -1347,26 +1349,35 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
 }
 
 // Touching (element, element) pairs of one wave, united in parallel rounds:
-// the pairs of several candidate chunks and cells are collected in LDS and
-// each lane then walks the union-find for one pair, so the dependent loads of
-// the walks overlap instead of adding up chunk after chunk.
+// the pairs of several candidate chunks and cells - of any streams of the
+// launch - are collected in LDS and each lane then walks the union-find of
+// its pair's stream, so the dependent loads of the walks overlap instead of
+// adding up cell after cell.
 constexpr int kEdgeCap = 256;
 
-struct EdgeBuf {
-    int2 *e;      // this wave's LDS slice
-    int n;        // pairs held (uniform)
-    int32_t *parent;  // union-find of the stream the pairs belong to
+struct Edge {
+    int x, y;  // union-find elements (slots)
+    int s;     // stream
 };
 
-__device__ __forceinline__ void edge_flush(EdgeBuf &eb) {
-    for (int i = lane_id(); i < eb.n; i += 64) uf_unite(eb.parent, eb.e[i].x, eb.e[i].y);
+struct EdgeBuf {
+    Edge *e;  // this wave's LDS slice
+    int n;    // pairs held (uniform)
+};
+
+__device__ __forceinline__ void edge_flush(const BatchArgs &a, EdgeBuf &eb) {
+    for (int i = lane_id(); i < eb.n; i += 64) {
+        const Edge e = eb.e[i];
+        uf_unite(ldc(a.s[e.s].parent), e.x, e.y);
+    }
     eb.n = 0;
 }
 
 // Queues (target, rep) for every lane that wants it, once per run of lanes
 // holding the same representative (a safe cell's points are adjacent in a
 // candidate chunk and share one representative).
-__device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int rep, int target, WaveStats &st) {
+__device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &eb, int s, bool want, int rep,
+                                                   int target, WaveStats &st) {
     const int lane = lane_id();
     const uint64_t m = __ballot(want);
     if (m == 0ull) return;
@@ -1378,8 +1389,8 @@ __device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int r
     const int nf = __popcll(fm);
     st.v[kStatMergePairs] += __popcll(m);
     st.v[kStatUnites] += nf;
-    if (eb.n + nf > kEdgeCap) edge_flush(eb);
-    if (first) eb.e[eb.n + __popcll(fm & ((1ull << lane) - 1ull))] = make_int2(target, rep);
+    if (eb.n + nf > kEdgeCap) edge_flush(a, eb);
+    if (first) eb.e[eb.n + __popcll(fm & ((1ull << lane) - 1ull))] = Edge{target, rep, s};
     eb.n += nf;
 }
 
@@ -1393,7 +1404,7 @@ __device__ __forceinline__ void unite_once_per_rep(EdgeBuf &eb, bool want, int r
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
-__device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
+__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
@@ -1467,7 +1478,7 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
                     const uint64_t t = __ballot(touched);
                     if (__ballot(!bcore || (t & run) != 0ull) == ~0ull) break;  // every candidate run settled
                 }
-                unite_once_per_rep(eb, touched, rep, repA, st);
+                unite_once_per_rep(a, eb, s, touched, rep, repA, st);
             } else {
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
@@ -1475,7 +1486,7 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
                     st.v[kStatMergeTests] += chunk;
                     const bool hit = bcore && pos < cs + q0 + qi &&
                                      sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
-                    unite_once_per_rep(eb, hit, rep, lane_i(qslot, qi), st);
+                    unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
                 }
             }
         }
@@ -1483,24 +1494,18 @@ __device__ void union_cell(const CellView &V, int A, WaveStats &st, EdgeBuf &eb)
 }
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
-    __shared__ int2 edges[kCellWaves][kEdgeCap];
+    __shared__ Edge edges[kCellWaves][kEdgeCap];
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     WaveStats st;
-    EdgeBuf eb{edges[threadIdx.x >> 6], 0, nullptr};
-    int es = -1;  // stream of the queued pairs
+    EdgeBuf eb{edges[threadIdx.x >> 6], 0};
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
         const int s = uni(x.s);
         stats_stream(a, st, s);
-        if (s != es) {
-            if (es >= 0) edge_flush(eb);
-            es = s;
-            eb.parent = ldc(a.s[s].parent);
-        }
-        union_cell(cell_view(a.s[s]), uni(x.c), st, eb);
+        union_cell(a, s, cell_view(a.s[s]), uni(x.c), st, eb);
     }
-    if (es >= 0) edge_flush(eb);
+    edge_flush(a, eb);
     stats_flush(a, st);
 }
 
@@ -1513,25 +1518,34 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
 template <int LPQ>
 __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
     const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
-    const StreamDev &S = a.s[ql.x.s];
+    const int s = uni(ql.x.s);
+    const CellView V = cell_view(a.s[s]);
     const int p = ql.p;
     const bool valid = p < ql.x.p1;
-    const int dim = S.dim, minpts = S.min_pts;
-    const float e2 = S.eps2;
+    const int dim = V.dim, minpts = V.min_pts;
+    const float e2 = V.eps2;
     int cnt = 0;
     bool dense = false;
     int c = 0;
+    // the query's cell and coordinates and its slot, one round trip
+    const int pp = valid ? p : ql.x.p0;
+    const int cv = V.pcell[pp];
+    const float qx = V.sx[pp];
+    const float qy = dim > 1 ? V.sy[pp] : 0.f;
+    const float qz = dim > 2 ? V.sz[pp] : 0.f;
+    const uint64_t qk = V.skey[pp];
+    arrived(qx);
+    arrived(qy);
+    arrived(qz);
+    arrived(qk);
     if (valid) {
-        c = S.pcell[p];
-        dense = S.cflag[c] & kCellDense;
+        c = cv;
+        dense = V.cflag[c] & kCellDense;
     }
     int tests = 0;
     if (valid && !dense) {
-        const float qx = S.sx[p];
-        const float qy = dim > 1 ? S.sy[p] : 0.f;
-        const float qz = dim > 2 ? S.sz[p] : 0.f;
         walk_candidates(
-            S, c,
+            V, c,
             [&](int, float cx, float cy, float cz) {
                 tests++;
                 cnt += sqd(dim, qx, qy, qz, cx, cy, cz) < e2;
@@ -1544,7 +1558,7 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
         const int nd = __popcll(__ballot(valid && dense && ql.sub == 0));
         const int nq = __popcll(__ballot(valid && !dense && ql.sub == 0));
         if (lane_id() == 0) {
-            unsigned long long *st = a.stats + (size_t)ql.x.s * kNumStats;
+            unsigned long long *st = a.stats + (size_t)s * kNumStats;
             atomicAdd(st + kStatCountTests, t);
             if (nd) atomicAdd(st + kStatDensePoints, (unsigned long long)nd);
             if (nq) atomicAdd(st + kStatQueries, (unsigned long long)nq);
@@ -1552,9 +1566,9 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
     }
     cnt = group_sum<LPQ>(cnt);
     if (valid && ql.sub == 0) {
-        const int sl = slot_of(S, p);
-        S.parent[sl] = sl;
-        S.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
+        const int sl = (int)((uint32_t)qk & V.smask);
+        V.parent[sl] = sl;
+        V.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
     }
 }
 
@@ -1568,39 +1582,48 @@ __global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
 template <int LPQ>
 __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
-    const StreamDev &S = a.s[ql.x.s];
+    const int s = uni(ql.x.s);
+    const CellView V = cell_view(a.s[s]);
     const int p = ql.p;
-    bool mine = p < ql.x.p1;
-    if (mine) mine = !(S.flags[p] & kFlagCore);
-    const int dim = S.dim;
-    const float e2 = S.eps2;
+    // the query's flags, cell, coordinates and slot, one round trip
+    const int pp = p < ql.x.p1 ? p : ql.x.p0;
+    const int qf = V.flags[pp];
+    const int c = V.pcell[pp];
+    const int dim = V.dim;
+    const float qx = V.sx[pp];
+    const float qy = dim > 1 ? V.sy[pp] : 0.f;
+    const float qz = dim > 2 ? V.sz[pp] : 0.f;
+    const uint64_t qk = V.skey[pp];
+    arrived(c);
+    arrived(qx);
+    arrived(qy);
+    arrived(qz);
+    arrived(qk);
+    const bool mine = p < ql.x.p1 && !(qf & kFlagCore);
+    const float e2 = V.eps2;
     int best = INT_MAX;
     int tests = 0;
     if (mine) {
-        const int c = S.pcell[p];
-        const float qx = S.sx[p];
-        const float qy = dim > 1 ? S.sy[p] : 0.f;
-        const float qz = dim > 2 ? S.sz[p] : 0.f;
         walk_candidates(
-            S, c,
+            V, c,
             [&](int j, float cx, float cy, float cz) {
                 tests++;
-                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
-                    best = min(best, S.canon[j]);
+                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (V.flags[j] & kFlagCore))
+                    best = min(best, V.canon[j]);
                 return false;
             },
             ql.sub, LPQ);
     }
     if (a.stats) {
         const unsigned long long t = (unsigned long long)wave_sum(tests);
-        if (lane_id() == 0 && t) atomicAdd(a.stats + (size_t)ql.x.s * kNumStats + kStatBorderTests, t);
+        if (lane_id() == 0 && t) atomicAdd(a.stats + (size_t)s * kNumStats + kStatBorderTests, t);
     }
     best = group_min<LPQ>(best);
     if (mine && ql.sub == 0) {
-        const IO io = a.use_io1 ? a.io1 : a.io[ql.x.s];
-        const int slot = slot_of(S, p);
-        io.label[slot] = best != INT_MAX ? 1 : 2;
-        io.cid[slot] = best != INT_MAX ? best : slot;
+        const IO io = a.use_io1 ? a.io1 : a.io[s];
+        const int slot = (int)((uint32_t)qk & V.smask);
+        gl(io.label)[slot] = best != INT_MAX ? 1 : 2;
+        gl(io.cid)[slot] = best != INT_MAX ? best : slot;
     }
 }
 
@@ -2075,7 +2098,7 @@ __device__ __forceinline__ void union_points_chunk(const BatchArgs &a, int g) {
     const float qy = dim > 1 ? S.sy[p] : 0.f;
     const float qz = dim > 2 ? S.sz[p] : 0.f;
     const int sp = slot_of(S, p);
-    walk_candidates(S, c, [&](int j, float cx, float cy, float cz) {
+    walk_candidates(cell_view(S), c, [&](int j, float cx, float cy, float cz) {
         if (j != p && sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (S.flags[j] & kFlagCore))
             uf_unite(S.parent, sp, rep_slot(S, j));
         return false;
@@ -2085,29 +2108,37 @@ __device__ __forceinline__ void union_points_chunk(const BatchArgs &a, int g) {
 __global__ __launch_bounds__(64) void k_union_points(BatchArgs a) { union_points_chunk(a, blockIdx.x); }
 
 // Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
-// root of the core's representative. Block 0 of each stream also advances the
-// ring cursor; no later kernel of the slide reads it.
-__device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g) {
+// root of the core's union-find element - read from rep[] when the merge kept
+// it (use_rep), else derived from the cell. Block 0 of each stream also
+// advances the ring cursor; no later kernel of the slide reads it.
+__device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g, int use_rep) {
     const ChunkCtx x = chunk_of(a, g);
-    const StreamDev &S = a.s[x.s];
-    const IO io = a.use_io1 ? a.io1 : a.io[x.s];
+    const int s = uni(x.s);
+    const StreamDev &S = a.s[s];
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int p = x.p0 + lane_id();
-    if (g == ldc(a.pre_chunk[x.s]) && lane_id() == 0) {
+    if (g == ldc(a.pre_chunk[s]) && lane_id() == 0) {
         StreamMut *m = S.mut;
-        if (a.stats) a.stats[(size_t)x.s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
+        if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
         m->t = m->t + 1;
         m->u = m->u + 1 == S.units ? 0 : m->u + 1;
     }
     if (p >= x.p1) return;
-    if (!(S.flags[p] & kFlagCore)) return;
-    const int v = uf_find(S.parent, rep_slot(S, p));  // minimum core slot of the cluster
-    S.canon[p] = v;
-    const int slot = slot_of(S, p);
-    io.label[slot] = 0;
-    io.cid[slot] = v;
+    const CellView V = cell_view(S);
+    const int f = V.flags[p];
+    const uint64_t k = V.skey[p];
+    const int rp = use_rep ? V.rep[p] : 0;
+    arrived(k);
+    arrived(rp);
+    if (!(f & kFlagCore)) return;
+    const int v = uf_find(ldc(S.parent), use_rep ? rp : rep_slot(S, p));  // minimum core slot of the cluster
+    V.canon[p] = v;
+    const int slot = (int)((uint32_t)k & V.smask);
+    gl(io.label)[slot] = 0;
+    gl(io.cid)[slot] = v;
 }
 
-__global__ __launch_bounds__(64) void k_flatten(BatchArgs a) { flatten_chunk(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_flatten(BatchArgs a, int use_rep) { flatten_chunk(a, blockIdx.x, use_rep); }
 
 // Plain device copy of 16-byte words: a kernel of known HBM traffic (reads and
 // writes `n16` * 16 bytes) used to scale the profiler's byte counters.
@@ -2197,7 +2228,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
-    RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a);
+    // rep[] holds every position's element unless the cell-level merge kernels ran
+    RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, (sh.cellpar || !sh.union_cells) ? 1 : 0);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
     if (sh.cellpar) RTDBS_LAUNCH(k_border, sh.cellpar_blocks, 64 * kCellWaves, a);
