@@ -634,7 +634,9 @@ __global__ __launch_bounds__(kTileThreads) void k_merge_final(BatchArgs a) {
 // Cell table: ordered compaction of run starts -> cstart/ckey and position ->
 // cell. Replaces compute_offsets_of_cells + thrust_sort
 // (aabb.cu:377-410, thrust_helper.cu:16-19): the order comes from the scan,
-// so no sort and no host round trip (dbscan.cpp:549) are needed.
+// so no sort and no host round trip (dbscan.cpp:549) are needed. Also resets
+// the union-find (every slot its own root) in index order, coalesced, rather
+// than slot by slot from the count kernels.
 __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
     __shared__ int lds4[4];
     const int g = blockIdx.x;
@@ -668,7 +670,10 @@ __global__ __launch_bounds__(kTileThreads) void k_cells(BatchArgs a) {
         }
         int tot;
         const int rank = block_rank_flags(f, &tot, lds4);
-        if (valid) S.pcell[p] = base + run + rank + (f ? 1 : 0) - 1;
+        if (valid) {
+            S.pcell[p] = base + run + rank + (f ? 1 : 0) - 1;
+            S.parent[p] = p;  // union-find over slots [0, W), each its own root; written in order here
+        }
         if (f) {
             const int c = base + run + rank;
             S.cstart[c] = p;
@@ -1201,10 +1206,8 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
         st.v[kStatDensePoints] += nA;
         const int rs = uni(vslot(V, cs));
         for (int i = lane; i < nA; i += 64) {
-            const int p = cs + i, sl = vslot(V, p);
-            V.parent[sl] = sl;
-            V.flags[p] = kFlagCore | kFlagDense;
-            V.rep[p] = rs;
+            V.flags[cs + i] = kFlagCore | kFlagDense;
+            V.rep[cs + i] = rs;
         }
         if (lane == 0) V.fcore[A] = cs;
         return;
@@ -1243,7 +1246,6 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
             open &= ~__ballot(mine >= minpts);
         }
         if (lane < nq) {
-            V.parent[qslot] = qslot;
             V.flags[cs + q0 + lane] = mine >= minpts ? kFlagCore : 0;
             if (!safe) V.rep[cs + q0 + lane] = qslot;  // unsafe cell: every core its own element
         }
@@ -1533,11 +1535,9 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
     const float qx = V.sx[pp];
     const float qy = dim > 1 ? V.sy[pp] : 0.f;
     const float qz = dim > 2 ? V.sz[pp] : 0.f;
-    const uint64_t qk = V.skey[pp];
     arrived(qx);
     arrived(qy);
     arrived(qz);
-    arrived(qk);
     if (valid) {
         c = cv;
         dense = V.cflag[c] & kCellDense;
@@ -1565,11 +1565,7 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
         }
     }
     cnt = group_sum<LPQ>(cnt);
-    if (valid && ql.sub == 0) {
-        const int sl = (int)((uint32_t)qk & V.smask);
-        V.parent[sl] = sl;
-        V.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
-    }
+    if (valid && ql.sub == 0) V.flags[p] = dense ? (kFlagCore | kFlagDense) : (cnt >= minpts ? kFlagCore : 0);
 }
 
 template <int LPQ>
