@@ -1112,11 +1112,21 @@ __device__ __forceinline__ float lane_f(float v, int l) {
 
 // Work counters of one wave (uniform values), flushed to the per-stream
 // totals with one atomic per counter when the wave moves to another stream
-// or ends. Only when counting is on (a.stats non-null).
+// or ends. The cell kernels are instantiated with WaveStats when counting is
+// on (a.stats non-null) and with NoStats otherwise, so that normal launches
+// carry no counting instructions.
 struct WaveStats {
     int s = -1;
     unsigned long long v[kNumStats] = {};
+    __device__ __forceinline__ void add(int k, unsigned long long x) { v[k] += x; }
 };
+
+struct NoStats {
+    __device__ __forceinline__ void add(int, unsigned long long) {}
+};
+
+__device__ __forceinline__ void stats_flush(const BatchArgs &, NoStats &) {}
+__device__ __forceinline__ void stats_stream(const BatchArgs &, NoStats &, int) {}
 
 __device__ __forceinline__ void stats_flush(const BatchArgs &a, WaveStats &w) {
     if (a.stats && w.s >= 0 && lane_id() == 0)
@@ -1193,7 +1203,8 @@ __device__ __forceinline__ int walk_chunk(RowWalk &w, int jb, int *end) {
 // stops once the count reaches min_pts. Loads are issued in three rounds -
 // the cell's header and rows, its queries, each candidate chunk - each
 // waited for as a whole.
-__device__ void count_cell(const CellView &V, int A, WaveStats &st) {
+template <class ST>
+__device__ void count_cell(const CellView &V, int A, ST &st) {
     const int lane = lane_id();
     const int dim = V.dim, minpts = V.min_pts;
     const float e2 = V.eps2;
@@ -1203,7 +1214,7 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
     const int cs = uni(csv), nA = uni(cev) - cs;
     const int cf = uni(cfv);
     if (cf & kCellDense) {
-        st.v[kStatDensePoints] += nA;
+        st.add(kStatDensePoints, nA);
         const int rs = uni(vslot(V, cs));
         for (int i = lane; i < nA; i += 64) {
             V.flags[cs + i] = kFlagCore | kFlagDense;
@@ -1215,7 +1226,7 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
     const bool safe = cf & kCellSafe;
     const RowWalk w0 = row_walk(rw, INT_MAX);
     int fc = kNoCore, fslot = 0;
-    st.v[kStatQueries] += nA;
+    st.add(kStatQueries, nA);
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1235,7 +1246,7 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
             const float cz = dim > 2 ? V.sz[pos] : 0.f;
             const bool v = jb + lane < end;
             uint64_t m = open;
-            st.v[kStatCountTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb);
+            st.add(kStatCountTests, (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb));
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1262,10 +1273,11 @@ __device__ void count_cell(const CellView &V, int A, WaveStats &st) {
         for (int i = lane; i < nA; i += 64) V.rep[cs + i] = fc != kNoCore ? fslot : 0;
 }
 
+template <class ST>
 __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
-    WaveStats st;
+    ST st;
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
         const int s = uni(x.s);
@@ -1279,7 +1291,8 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
 // deterministic rule: the minimum canonical id among cores within eps) for
 // the non-core points of a cell; cells without them are skipped. Loads in
 // rounds, as in count_cell.
-__device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &st) {
+template <class ST>
+__device__ void border_cell(const CellView &V, const IO &io, int A, ST &st) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
@@ -1319,7 +1332,7 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &s
             const bool core = jb + lane < end && (f & kFlagCore);
             const int cid = core ? id : INT_MAX;
             uint64_t m = open;
-            st.v[kStatBorderTests] += (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb);
+            st.add(kStatBorderTests, (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb));
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
@@ -1337,10 +1350,11 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, WaveStats &s
     }
 }
 
+template <class ST>
 __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
-    WaveStats st;
+    ST st;
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
         const int s = uni(x.s);
@@ -1378,8 +1392,9 @@ __device__ __forceinline__ void edge_flush(const BatchArgs &a, EdgeBuf &eb) {
 // Queues (target, rep) for every lane that wants it, once per run of lanes
 // holding the same representative (a safe cell's points are adjacent in a
 // candidate chunk and share one representative).
+template <class ST>
 __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &eb, int s, bool want, int rep,
-                                                   int target, WaveStats &st) {
+                                                   int target, ST &st) {
     const int lane = lane_id();
     const uint64_t m = __ballot(want);
     if (m == 0ull) return;
@@ -1389,8 +1404,8 @@ __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &
     const bool first = want && (below == 0ull || prep != rep);
     const uint64_t fm = __ballot(first);
     const int nf = __popcll(fm);
-    st.v[kStatMergePairs] += __popcll(m);
-    st.v[kStatUnites] += nf;
+    st.add(kStatMergePairs, __popcll(m));
+    st.add(kStatUnites, nf);
     if (eb.n + nf > kEdgeCap) edge_flush(a, eb);
     if (first) eb.e[eb.n + __popcll(fm & ((1ull << lane) - 1ull))] = Edge{target, rep, s};
     eb.n += nf;
@@ -1406,7 +1421,8 @@ __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
-__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, WaveStats &st, EdgeBuf &eb) {
+template <class ST>
+__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, ST &st, EdgeBuf &eb) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
@@ -1474,7 +1490,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
-                    st.v[kStatMergeTests] += chunk;
+                    st.add(kStatMergeTests, chunk);
                     touched = touched ||
                               (bcore && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                     const uint64_t t = __ballot(touched);
@@ -1485,7 +1501,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
-                    st.v[kStatMergeTests] += chunk;
+                    st.add(kStatMergeTests, chunk);
                     const bool hit = bcore && pos < cs + q0 + qi &&
                                      sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
                     unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
@@ -1495,11 +1511,12 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
     }
 }
 
+template <class ST>
 __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     __shared__ Edge edges[kCellWaves][kEdgeCap];
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
-    WaveStats st;
+    ST st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0};
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
         const CellCtx x = cell_of(a, gc);
@@ -2203,7 +2220,9 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         // their own between it and its neighbours.
         const bool timed = ev && ((phase_mask >> kPhCount) & 1u);
         const hipEvent_t e0 = timed ? ev[2 * kPhCount] : nullptr, e1 = timed ? ev[2 * kPhCount + 1] : nullptr;
-        if (sh.cellpar) e = launch_timed(k_count, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
+        if (sh.cellpar && a.stats)
+            e = launch_timed(k_count<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
+        else if (sh.cellpar) e = launch_timed(k_count<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
         else if (sh.lpq >= 4) e = launch_timed(k_count_points<4>, sh.chunks * 4, 64, st, e0, e1, a);
         else if (sh.lpq == 2) e = launch_timed(k_count_points<2>, sh.chunks * 2, 64, st, e0, e1, a);
         else e = launch_timed(k_count_points<1>, sh.chunks, 64, st, e0, e1, a);
@@ -2211,7 +2230,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     }
     PH_BEGIN(kPhUnion);
     if (sh.cellpar) {
-        RTDBS_LAUNCH(k_union, sh.cellpar_blocks, 64 * kCellWaves, a);
+        if (a.stats) RTDBS_LAUNCH(k_union<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+        else RTDBS_LAUNCH(k_union<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     } else if (sh.union_cells) {
         RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
         RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
@@ -2220,7 +2240,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         // point-parallel count, cell-parallel merge
         RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
         RTDBS_LAUNCH(k_rep, sh.wtiles, kTileThreads, a);
-        RTDBS_LAUNCH(k_union, sh.cellpar_blocks, 64 * kCellWaves, a);
+        if (a.stats) RTDBS_LAUNCH(k_union<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+        else RTDBS_LAUNCH(k_union<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
@@ -2228,7 +2249,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, (sh.cellpar || !sh.union_cells) ? 1 : 0);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
-    if (sh.cellpar) RTDBS_LAUNCH(k_border, sh.cellpar_blocks, 64 * kCellWaves, a);
+    if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+    else if (sh.cellpar) RTDBS_LAUNCH(k_border<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.lpq >= 4) RTDBS_LAUNCH(k_border_points<4>, sh.chunks * 4, 64, a);
     else if (sh.lpq == 2) RTDBS_LAUNCH(k_border_points<2>, sh.chunks * 2, 64, a);
     else RTDBS_LAUNCH(k_border_points<1>, sh.chunks, 64, a);
