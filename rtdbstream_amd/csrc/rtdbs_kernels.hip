@@ -1251,7 +1251,7 @@ __device__ void count_cell(const CellView &V, int A, ST &st) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const int c =
-                    __popcll(__ballot(v && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2));
+                    __popcll(__ballot(v & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2)));
                 if (lane == qi) mine += c;
             }
             open &= ~__ballot(mine >= minpts);
@@ -1336,7 +1336,7 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, ST &st) {
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
-                const bool hit = core && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
+                const bool hit = core & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                 if (__ballot(hit) == 0ull) continue;
                 const int mn = wave_min(hit ? cid : INT_MAX);
                 if (lane == qi) best = min(best, mn);
@@ -1491,10 +1491,10 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
                     st.add(kStatMergeTests, chunk);
-                    touched = touched ||
-                              (bcore && sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
+                    // bitwise: every lane evaluates the test, no branch per query
+                    touched |= bcore & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                     const uint64_t t = __ballot(touched);
-                    if (__ballot(!bcore || (t & run) != 0ull) == ~0ull) break;  // every candidate run settled
+                    if (__ballot(!bcore | ((t & run) != 0ull)) == ~0ull) break;  // every candidate run settled
                 }
                 unite_once_per_rep(a, eb, s, touched, rep, repA, st);
             } else {
@@ -1502,8 +1502,8 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
                     st.add(kStatMergeTests, chunk);
-                    const bool hit = bcore && pos < cs + q0 + qi &&
-                                     sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2;
+                    const bool hit = bcore & (pos < cs + q0 + qi) &
+                                     (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                     unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
                 }
             }
