@@ -485,8 +485,6 @@ static int batch_alloc(rtdbs_batch *b) {
     b->d_stats = (unsigned long long *)(b->dmem + desc_b + io_b + pre_b);
     HIPCHK(hipHostMalloc((void **)&b->h_io, (size_t)n * sizeof(IO), hipHostMallocDefault));
     HIPCHK(hipStreamCreateWithFlags(&b->hs, hipStreamNonBlocking));
-    for (int i = 0; i < 2 * kNumPhases; i++) HIPCHK(hipEventCreate(&b->ev[i]));
-    b->ev_ready = true;
     return RTDBS_OK;
 }
 
@@ -526,9 +524,8 @@ void batch_free(rtdbs_batch *b) {
     if (b->hs) (void)hipStreamSynchronize(b->hs);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    if (b->ev_ready)
-        for (int i = 0; i < 2 * kNumPhases; i++)
-            if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+    for (hipEvent_t e : b->ev_ring)
+        if (e) (void)hipEventDestroy(e);
     if (b->hs) (void)hipStreamDestroy(b->hs);
     if (b->h_io) (void)hipHostFree(b->h_io);
     if (b->dmem) (void)hipFree(b->dmem);
@@ -656,7 +653,18 @@ int ensure_solo(rtdbs_stream *s) {
     return RTDBS_OK;
 }
 
-static void timing_begin(rtdbs_batch *b) { (void)b; }
+// Event set for the next timed slide (reads the full ring first).
+static int timing_events(rtdbs_batch *b, hipEvent_t **ev) {
+    *ev = nullptr;
+    if (!b->timing) return RTDBS_OK;
+    if (b->ring_used == kTimingRing) {
+        int rc = batch_collect_timing(b);
+        if (rc) return rc;
+    }
+    *ev = b->ev_ring.data() + (size_t)b->ring_used * 2 * kNumPhases;
+    b->ring_used++;
+    return RTDBS_OK;
+}
 
 int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *cid) {
     int rc = batch_check_current(b);
@@ -666,39 +674,36 @@ int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *
     a.io1.in = in;
     a.io1.label = label;
     a.io1.cid = cid;
-    if (b->timing && b->pending_timing) {
-        rc = batch_collect_timing(b);
-        if (rc) return rc;
-    }
-    timing_begin(b);
-    HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr, b->timing_mask));
-    b->pending_timing = b->timing;
+    hipEvent_t *ev = nullptr;
+    rc = timing_events(b, &ev);
+    if (rc) return rc;
+    HIPCHK(launch_slide(a, b->shape, b->hs, ev, b->timing_mask));
     for (auto *s : b->streams) s->slides++;
     if (b->stats_on) b->stats_slides++;
     return RTDBS_OK;
 }
 
+// Adds the elapsed times of every recorded slide to the totals.
 int batch_collect_timing(rtdbs_batch *b) {
-    if (!b->pending_timing) return RTDBS_OK;
-    for (int ph = kNumPhases - 1; ph >= 0; ph--)
-        if ((b->timing_mask >> ph) & 1u) {
-            HIPCHK(hipEventSynchronize(b->ev[2 * ph + 1]));
-            break;
+    if (b->ring_used == 0) return RTDBS_OK;
+    HIPCHK(hipStreamSynchronize(b->hs));
+    for (int k = 0; k < b->ring_used; k++) {
+        const hipEvent_t *ev = b->ev_ring.data() + (size_t)k * 2 * kNumPhases;
+        for (int ph = 0; ph < kNumPhases; ph++) {
+            if (!((b->timing_mask >> ph) & 1u)) continue;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[2 * ph], ev[2 * ph + 1]));
+            b->acc_ms[ph] += ms;
+            b->acc_n[ph]++;
         }
-    for (int ph = 0; ph < kNumPhases; ph++) {
-        if (!((b->timing_mask >> ph) & 1u)) continue;
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev[2 * ph], b->ev[2 * ph + 1]));
-        b->acc_ms[ph] += ms;
-        b->acc_n[ph]++;
     }
-    b->pending_timing = false;
+    b->ring_used = 0;
     return RTDBS_OK;
 }
 
 int batch_sync(rtdbs_batch *b) {
     HIPCHK(hipStreamSynchronize(b->hs));
-    return batch_collect_timing(b);
+    return RTDBS_OK;
 }
 
 // One slide of all member streams with device I/O (table in d_io).
@@ -733,14 +738,11 @@ int batch_launch(rtdbs_batch *b, const IO *io) {
             HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
         }
         HIPCHK(hipGraphLaunch(b->gexec, b->hs));
-        b->pending_timing = false;
     } else {
-        if (b->timing && b->pending_timing) {
-            rc = batch_collect_timing(b);
-            if (rc) return rc;
-        }
-        HIPCHK(launch_slide(a, b->shape, b->hs, b->timing ? b->ev : nullptr, b->timing_mask));
-        b->pending_timing = b->timing;
+        hipEvent_t *ev = nullptr;
+        rc = timing_events(b, &ev);
+        if (rc) return rc;
+        HIPCHK(launch_slide(a, b->shape, b->hs, ev, b->timing_mask));
     }
     for (auto *s : b->streams) s->slides++;
     if (b->stats_on) b->stats_slides++;
@@ -875,6 +877,11 @@ extern "C" int rtdbs_batch_set_timing(rtdbs_batch *b, int enable) {
     if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
     int rc = batch_sync(b);
     if (rc) return rc;
+    b->ring_used = 0;  // recorded slides of the previous setting are dropped
+    if (enable && b->ev_ring.empty()) {
+        b->ev_ring.assign((size_t)kTimingRing * 2 * kNumPhases, nullptr);
+        for (hipEvent_t &e : b->ev_ring) HIPCHK(hipEventCreate(&e));
+    }
     // 1: every phase; 2: the range-count kernel only (two events per slide)
     b->timing = enable != 0;
     b->timing_mask = enable == 2 ? (1u << kPhCount) : (enable ? (1u << kNumPhases) - 1u : 0u);
@@ -895,7 +902,7 @@ extern "C" int rtdbs_stream_set_timing(rtdbs_stream *s, int enable) {
 extern "C" int rtdbs_batch_get_timing(rtdbs_batch *b, double *ms, int64_t *launches, int max_phases, char *names,
                                       size_t names_len) {
     if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
-    int rc = batch_sync(b);
+    int rc = batch_collect_timing(b);
     if (rc) return rc;
     const int np = std::min(max_phases, (int)kNumPhases);
     for (int i = 0; i < np; i++) {

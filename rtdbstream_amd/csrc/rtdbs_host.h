@@ -10,6 +10,10 @@
 
 struct rtdbs_batch;
 
+namespace rtdbs {
+constexpr int kTimingRing = 64;  // timed slides recorded before their events are read
+}
+
 struct rtdbs_stream {
     rtdbs_params p{};
     int k = 0, R = 0, slot_bits = 0, dense_ok = 0;
@@ -54,9 +58,12 @@ struct rtdbs_batch {
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
     uint32_t timing_mask = 0;  // phases bracketed by events
-    bool pending_timing = false;
-    bool ev_ready = false;
-    hipEvent_t ev[2 * rtdbs::kNumPhases] = {};
+    // Event sets of the slides recorded since the last read, 2 * kNumPhases
+    // per slide, created when timing is first switched on. The elapsed times
+    // are read only when the timings are read or the ring is full, so a timed
+    // slide costs no host calls beyond its launch.
+    std::vector<hipEvent_t> ev_ring;
+    int ring_used = 0;
     double acc_ms[rtdbs::kNumPhases] = {};
     int64_t acc_n[rtdbs::kNumPhases] = {};
 };
