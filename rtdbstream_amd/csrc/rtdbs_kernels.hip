@@ -240,15 +240,23 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *ld
 
 // Loads the packed keys (cell << slot_bits | slot) of points [tile0, tile0 +
 // N) of `in` (len points, slots from slot0) into v, 8 per thread (tid < N/8),
-// and sorts them; padding keys are ~0 and sort last.
+// and sorts them; padding keys are ~0 and sort last. Thread tid takes points
+// tid, tid + N/8, ... so each load instruction reads consecutive points (the
+// network sorts any assignment; a key carries its own slot).
 template <int N>
 __device__ __forceinline__ void sort_tile_keys(const StreamDev &S, const float *in, int len, int tile0, int slot0,
                                                uint64_t (&v)[8], uint64_t *lds, int tid) {
     const int dim = S.dim;
+    float c[8][3];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const int gi = tile0 + tid * 8 + e;
-        v[e] = gi < len ? (cell_key_of(S, in + (size_t)gi * dim) << S.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
+        const int gi = min(tile0 + e * (N / 8) + tid, len - 1);
+        for (int a = 0; a < 3; a++) c[e][a] = a < dim ? in[(size_t)gi * dim + a] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int gi = tile0 + e * (N / 8) + tid;
+        v[e] = gi < len ? (cell_key_of(S, c[e]) << S.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
     }
     bitonic_sort_regs<N>(v, lds, tid);
 }
