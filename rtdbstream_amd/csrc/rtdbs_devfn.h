@@ -81,10 +81,29 @@ __device__ __forceinline__ int uf_find(int32_t *parent, int x) {
     return x;
 }
 
+// The roots of a and b, walked together (uf_find's path halving on both
+// paths) so that the loads of both walks are in flight at once.
+__device__ __forceinline__ void uf_find2(int32_t *parent, int &a, int &b) {
+    int pa = uf_load(parent + a), pb = uf_load(parent + b);
+    while (pa != a || pb != b) {
+        const int ga = pa != a ? uf_load(parent + pa) : a;
+        const int gb = pb != b ? uf_load(parent + pb) : b;
+        if (pa != a) {
+            if (ga != pa) atomicMin(parent + a, ga);
+            a = pa;
+            pa = ga;
+        }
+        if (pb != b) {
+            if (gb != pb) atomicMin(parent + b, gb);
+            b = pb;
+            pb = gb;
+        }
+    }
+}
+
 __device__ __forceinline__ void uf_unite(int32_t *parent, int a, int b) {
     while (true) {
-        a = uf_find(parent, a);
-        b = uf_find(parent, b);
+        uf_find2(parent, a, b);
         if (a == b) return;
         if (a > b) {
             const int t = a;
