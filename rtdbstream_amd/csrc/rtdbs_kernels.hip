@@ -1295,34 +1295,64 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
     stats_flush(a, st);
 }
 
-// Border / noise (the reference's border claim, dbscan.cu:370-377, with the
-// deterministic rule: the minimum canonical id among cores within eps) for
-// the non-core points of a cell; cells without them are skipped. Loads in
-// rounds, as in count_cell.
+// Root of x's union-find tree, read only (no path halving): for kernels that
+// run after the merge has finished, when the trees no longer change.
+__device__ __forceinline__ int uf_root(const int32_t *parent, int x) {
+    int p = uf_load(parent + x);
+    while (p != x) {
+        x = p;
+        p = uf_load(parent + x);
+    }
+    return x;
+}
+
+// Outputs of one cell, after the merge (the reference's post_cluster_t,
+// aabb.cu:173-196, and border claim, dbscan.cu:370-377): a core takes the
+// root of its union-find element - the minimum core slot of its cluster; a
+// non-core point takes the minimum root among the cores within eps (border,
+// the deterministic form of the reference's claim) or stays noise. The roots
+// are found here, so no separate flatten pass runs. The wave of each
+// stream's first cell also advances the ring cursor. Loads in rounds, as in
+// count_cell.
 template <class ST>
-__device__ void border_cell(const CellView &V, const IO &io, int A, ST &st) {
+__device__ void output_cell(const BatchArgs &a, int s, const CellView &V, const IO &io, int A, ST &st) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
+    const int32_t *parent = ldc(a.s[s].parent);
     const int cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
     const uint64_t rw = row_word(V, A);
     arrived(rw);
-    if (uni(cfv) & kCellDense) return;
+    if (A == 0 && lane == 0) {
+        StreamMut *m = ldc(a.s[s].mut);
+        if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
+        m->t = m->t + 1;
+        m->u = m->u + 1 == ldc(a.s[s].units) ? 0 : m->u + 1;
+    }
     const int cs = uni(csv), nA = uni(cev) - cs;
+    const bool dense = uni(cfv) & kCellDense;
     gptr<int32_t> label = gl(io.label), cidv = gl(io.cid);
     const RowWalk w0 = row_walk(rw, INT_MAX);
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
         const int qf = V.flags[pq];
+        const uint64_t qk = V.skey[pq];
+        const int qr = V.rep[pq];
         const float qx = V.sx[pq];
         const float qy = dim > 1 ? V.sy[pq] : 0.f;
         const float qz = dim > 2 ? V.sz[pq] : 0.f;
-        const uint64_t qk = V.skey[pq];
+        arrived(qk);
+        arrived(qr);
         arrived(qx);
         arrived(qy);
         arrived(qz);
-        arrived(qk);
+        const int qslot = (int)((uint32_t)qk & V.smask);
+        if (lane < nq && (qf & kFlagCore)) {
+            label[qslot] = 0;
+            cidv[qslot] = uf_root(parent, qr);
+        }
+        if (dense) continue;
         const bool isq = lane < nq && !(qf & kFlagCore);
         const uint64_t open = __ballot(isq);
         if (open == 0ull) continue;
@@ -1336,24 +1366,33 @@ __device__ void border_cell(const CellView &V, const IO &io, int A, ST &st) {
             const float cx = V.sx[pos];
             const float cy = dim > 1 ? V.sy[pos] : 0.f;
             const float cz = dim > 2 ? V.sz[pos] : 0.f;
-            const int id = V.canon[pos];  // meaningful for cores only
+            const int rp = V.rep[pos];  // meaningful for cores only
+            arrived(cx);
+            arrived(cy);
+            arrived(cz);
+            arrived(rp);
             const bool core = jb + lane < end && (f & kFlagCore);
-            const int cid = core ? id : INT_MAX;
             uint64_t m = open;
             st.add(kStatBorderTests, (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb));
+            // roots of the chunk's cores, found once a query of the cell hits one
+            int cid = INT_MAX;
+            bool have = false;
             while (m) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const bool hit = core & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
                 if (__ballot(hit) == 0ull) continue;
+                if (!have) {
+                    if (core) cid = uf_root(parent, rp);
+                    have = true;
+                }
                 const int mn = wave_min(hit ? cid : INT_MAX);
                 if (lane == qi) best = min(best, mn);
             }
         }
         if (isq) {
-            const int slot = (int)((uint32_t)qk & V.smask);
-            label[slot] = best != INT_MAX ? 1 : 2;
-            cidv[slot] = best != INT_MAX ? best : slot;
+            label[qslot] = best != INT_MAX ? 1 : 2;
+            cidv[qslot] = best != INT_MAX ? best : qslot;
         }
     }
 }
@@ -1367,7 +1406,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
         const CellCtx x = cell_of(a, gc);
         const int s = uni(x.s);
         stats_stream(a, st, s);
-        border_cell(cell_view(a.s[s]), a.use_io1 ? a.io1 : a.io[s], uni(x.c), st);
+        output_cell(a, s, cell_view(a.s[s]), a.use_io1 ? a.io1 : a.io[s], uni(x.c), st);
     }
     stats_flush(a, st);
 }
@@ -2253,8 +2292,10 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
-    // rep[] holds every position's element unless the cell-level merge kernels ran
-    RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, (sh.cellpar || !sh.union_cells) ? 1 : 0);
+    // the cell-parallel output kernel finds the roots itself; the point form
+    // reads the cores' canonical ids written here (from rep[] unless the
+    // cell-level merge kernels ran)
+    if (!sh.cellpar) RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, sh.union_cells ? 0 : 1);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
     if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
