@@ -1638,8 +1638,12 @@ __global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
 }
 
 // Border / noise, point-parallel form (large batches): LPQ lanes per
-// non-core point.
-template <int LPQ>
+// non-core point. With ROOTS (rep[] holds every position's element) the
+// kernel also writes the cores' outputs, takes candidate roots from the
+// union-find itself and advances each stream's ring cursor, as the
+// cell-parallel output kernel does; without it (the cell-level merge ran) it
+// reads the canonical ids that k_flatten wrote.
+template <int LPQ, bool ROOTS>
 __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
     const int s = uni(ql.x.s);
@@ -1654,11 +1658,28 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const float qy = dim > 1 ? V.sy[pp] : 0.f;
     const float qz = dim > 2 ? V.sz[pp] : 0.f;
     const uint64_t qk = V.skey[pp];
+    const int qr = ROOTS ? V.rep[pp] : 0;
     arrived(c);
     arrived(qx);
     arrived(qy);
     arrived(qz);
     arrived(qk);
+    arrived(qr);
+    const int32_t *parent = ldc(a.s[s].parent);
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    if (ROOTS) {
+        if (vb == ldc(a.pre_chunk[s]) * LPQ && lane_id() == 0) {  // the stream's first block
+            StreamMut *m = ldc(a.s[s].mut);
+            if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
+            m->t = m->t + 1;
+            m->u = m->u + 1 == ldc(a.s[s].units) ? 0 : m->u + 1;
+        }
+        if (p < ql.x.p1 && (qf & kFlagCore) && ql.sub == 0) {
+            const int slot = (int)((uint32_t)qk & V.smask);
+            gl(io.label)[slot] = 0;
+            gl(io.cid)[slot] = uf_root(parent, qr);
+        }
+    }
     const bool mine = p < ql.x.p1 && !(qf & kFlagCore);
     const float e2 = V.eps2;
     int best = INT_MAX;
@@ -1669,7 +1690,7 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
             [&](int j, float cx, float cy, float cz) {
                 tests++;
                 if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (V.flags[j] & kFlagCore))
-                    best = min(best, V.canon[j]);
+                    best = min(best, ROOTS ? uf_root(parent, V.rep[j]) : V.canon[j]);
                 return false;
             },
             ql.sub, LPQ);
@@ -1680,16 +1701,15 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     }
     best = group_min<LPQ>(best);
     if (mine && ql.sub == 0) {
-        const IO io = a.use_io1 ? a.io1 : a.io[s];
         const int slot = (int)((uint32_t)qk & V.smask);
         gl(io.label)[slot] = best != INT_MAX ? 1 : 2;
         gl(io.cid)[slot] = best != INT_MAX ? best : slot;
     }
 }
 
-template <int LPQ>
+template <int LPQ, bool ROOTS>
 __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
-    border_vblock<LPQ>(a, blockIdx.x);
+    border_vblock<LPQ, ROOTS>(a, blockIdx.x);
 }
 
 // First (minimum-slot) core of every cell - its union-find representative -
@@ -2292,17 +2312,21 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     }
     PH_END(kPhUnion);
     PH_BEGIN(kPhFlatten);
-    // the cell-parallel output kernel finds the roots itself; the point form
-    // reads the cores' canonical ids written here (from rep[] unless the
-    // cell-level merge kernels ran)
-    if (!sh.cellpar) RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, sh.union_cells ? 0 : 1);
+    // the output kernels find the roots themselves, except after the
+    // cell-level merge kernels (rep[] is not kept then): flatten writes the
+    // cores' canonical ids for them
+    if (!sh.cellpar && sh.union_cells) RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, 0);
     PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
     if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.cellpar) RTDBS_LAUNCH(k_border<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
-    else if (sh.lpq >= 4) RTDBS_LAUNCH(k_border_points<4>, sh.chunks * 4, 64, a);
-    else if (sh.lpq == 2) RTDBS_LAUNCH(k_border_points<2>, sh.chunks * 2, 64, a);
-    else RTDBS_LAUNCH(k_border_points<1>, sh.chunks, 64, a);
+    else if (sh.union_cells) {
+        if (sh.lpq >= 4) RTDBS_LAUNCH((k_border_points<4, false>), sh.chunks * 4, 64, a);
+        else if (sh.lpq == 2) RTDBS_LAUNCH((k_border_points<2, false>), sh.chunks * 2, 64, a);
+        else RTDBS_LAUNCH((k_border_points<1, false>), sh.chunks, 64, a);
+    } else if (sh.lpq >= 4) RTDBS_LAUNCH((k_border_points<4, true>), sh.chunks * 4, 64, a);
+    else if (sh.lpq == 2) RTDBS_LAUNCH((k_border_points<2, true>), sh.chunks * 2, 64, a);
+    else RTDBS_LAUNCH((k_border_points<1, true>), sh.chunks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END
