@@ -401,35 +401,51 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
     const int gb = g / kRankParts, part = g % kRankParts;
     const int s = find_stream_c(a.pre_stile, a.n, gb);
     const StreamDev &S = a.s[s];
+    // the stream's fields as constant memory and its arrays as global memory:
+    // global loads wait on their own counter, so the next stage's keys load
+    // while the LDS searches of this stage run
+    const int SS = ldc(S.S), dim = ldc(S.dim);
+    const gptr<const uint64_t> bkey0 = gl((const uint64_t *)ldc(S.bkey0));
     const int i = (gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads + threadIdx.x;
-    const bool valid = i < S.S;
-    const uint64_t key = valid ? S.bkey0[i] : ~0ull;
-    const int myrun = valid ? i / kRun : -1, nr = (S.S + kRun - 1) / kRun;
+    const bool valid = i < SS;
+    const uint64_t key = valid ? bkey0[i] : ~0ull;
+    const int myrun = valid ? i / kRun : -1, nr = (SS + kRun - 1) / kRun;
     int rank = valid ? i - myrun * kRun : 0;
     // the runs other than this block's own ones are staged; a block's keys
     // share at most one run (kTileThreads divides kRun)
     const int blockrun = ((gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads) / kRun;
-    for (int o0 = 0; o0 < nr - 1; o0 += kStageRuns) {
-        int len[kStageRuns];
-        constexpr int per = kStageRuns * kRun / kTileThreads;
-        uint64_t stage[per];  // every load issued before the LDS stores
-#pragma unroll
-        for (int q = 0; q < kStageRuns; q++) {
-            const int o = o0 + q;
-            const int r = o < blockrun ? o : o + 1;  // the o-th run other than the block's
-            len[q] = o < nr - 1 ? min(kRun, S.S - r * kRun) : 0;
-        }
+    constexpr int per = kStageRuns * kRun / kTileThreads;
+    // keys of the staged runs o0 .. o0 + kStageRuns - 1, ~0 past their ends
+    auto load_stage = [&](int o0, uint64_t(&st)[per]) {
 #pragma unroll
         for (int e = 0; e < per; e++) {
             const int slot = threadIdx.x + e * kTileThreads;  // [0, kStageRuns * kRun)
             const int q = slot / kRun, j = slot % kRun;
             const int o = o0 + q;
+            const int r = o < blockrun ? o : o + 1;  // the o-th run other than the block's
+            st[e] = (o < nr - 1 && j < min(kRun, SS - r * kRun)) ? bkey0[r * kRun + j] : ~0ull;
+        }
+    };
+    uint64_t cur[per], nxt[per];
+    if (nr > 1) load_stage(0, cur);
+    // the point's coordinates are needed only at the end: loaded now
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    const int gi = valid ? (int)((uint32_t)key & slot_mask(S)) - ldc(ldc(S.mut)->u) * SS : 0;
+    const gptr<const float> p = gl(io.in) + (size_t)gi * dim;
+    const float px = p[0], py = dim > 1 ? p[1] : 0.f, pz = dim > 2 ? p[2] : 0.f;
+    for (int o0 = 0; o0 < nr - 1; o0 += kStageRuns) {
+        const bool more = o0 + kStageRuns < nr - 1;
+        if (more) load_stage(o0 + kStageRuns, nxt);
+        int len[kStageRuns];
+#pragma unroll
+        for (int q = 0; q < kStageRuns; q++) {
+            const int o = o0 + q;
             const int r = o < blockrun ? o : o + 1;
-            stage[e] = (o < nr - 1 && j < min(kRun, S.S - r * kRun)) ? S.bkey0[r * kRun + j] : ~0ull;
+            len[q] = o < nr - 1 ? min(kRun, SS - r * kRun) : 0;
         }
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < per; e++) lds[threadIdx.x + e * kTileThreads] = stage[e];
+        for (int e = 0; e < per; e++) lds[threadIdx.x + e * kTileThreads] = cur[e];
         __syncthreads();
         int pos[kStageRuns];
 #pragma unroll
@@ -445,17 +461,16 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
         }
 #pragma unroll
         for (int q = 0; q < kStageRuns; q++) rank += pos[q];
+        if (more) {
+#pragma unroll
+            for (int e = 0; e < per; e++) cur[e] = nxt[e];
+        }
     }
     if (!valid) return;
-    const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int dim = S.dim;
-    const int gi = (int)((uint32_t)key & slot_mask(S)) - S.mut->u * S.S;
-    const float *p = io.in + (size_t)gi * dim;
-    const float px = p[0], py = dim > 1 ? p[1] : 0.f, pz = dim > 2 ? p[2] : 0.f;
-    S.bkey1[rank] = key;
-    S.bx1[rank] = px;
-    if (dim > 1) S.by1[rank] = py;
-    if (dim > 2) S.bz1[rank] = pz;
+    gl(S.bkey1)[rank] = key;
+    gl(S.bx1)[rank] = px;
+    if (dim > 1) gl(S.by1)[rank] = py;
+    if (dim > 2) gl(S.bz1)[rank] = pz;
 }
 
 // Launch 3: ordered compaction of the window's survivors.
