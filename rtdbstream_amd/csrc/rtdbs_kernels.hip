@@ -148,20 +148,45 @@ __device__ __forceinline__ int slot_of(const StreamDev &S, int pos) {
 // ---------------------------------------------------------------------------
 // grid
 
+// The grid of a stream read once as constant memory (uniform per block), for
+// kernels that compute many keys.
+struct KeyGrid {
+    double lo0, lo1, lo2, inv_side;
+    int64_t n0, n1, n2;
+    int dim, slot_bits;
+};
+
+__device__ __forceinline__ KeyGrid key_grid(const StreamDev &S) {
+    KeyGrid g;
+    g.lo0 = ldc(S.lo[0]);
+    g.lo1 = ldc(S.lo[1]);
+    g.lo2 = ldc(S.lo[2]);
+    g.inv_side = ldc(S.inv_side);
+    g.n0 = ldc(S.n[0]);
+    g.n1 = ldc(S.n[1]);
+    g.n2 = ldc(S.n[2]);
+    g.dim = ldc(S.dim);
+    g.slot_bits = ldc(S.slot_bits);
+    return g;
+}
+
 // Cell index per axis: floor((x - lo) / side) evaluated in double and clamped
 // into the grid. Clamping keeps the search exact: |clamp(i) - clamp(j)| <=
 // |i - j|, so points within reach stay within reach.
-__device__ __forceinline__ uint64_t cell_key_of(const StreamDev &S, const float *p) {
-    int64_t ix[3] = {0, 0, 0};
-    for (int a = 0; a < S.dim; a++) {
-        double v = ((double)p[a] - S.lo[a]) * S.inv_side;
-        v = floor(v);
-        if (!(v >= 0.0)) v = 0.0;  // negative or NaN
-        const double mx = (double)(S.n[a] - 1);
-        if (v > mx) v = mx;
-        ix[a] = (int64_t)v;
-    }
-    return (uint64_t)((ix[0] * S.n[1] + ix[1]) * S.n[2] + ix[2]);
+__device__ __forceinline__ int64_t bin_axis(double x, double lo, double inv, int64_t n) {
+    double v = floor((x - lo) * inv);
+    if (!(v >= 0.0)) v = 0.0;  // negative or NaN
+    const double mx = (double)(n - 1);
+    if (v > mx) v = mx;
+    return (int64_t)v;
+}
+
+// Packed cell key (i0 * n1 + i1) * n2 + i2; unused axes have index 0.
+__device__ __forceinline__ uint64_t cell_key_g(const KeyGrid &G, float x, float y, float z) {
+    const int64_t i0 = bin_axis((double)x, G.lo0, G.inv_side, G.n0);
+    const int64_t i1 = G.dim > 1 ? bin_axis((double)y, G.lo1, G.inv_side, G.n1) : 0;
+    const int64_t i2 = G.dim > 2 ? bin_axis((double)z, G.lo2, G.inv_side, G.n2) : 0;
+    return (uint64_t)((i0 * G.n1 + i1) * G.n2 + i2);
 }
 
 // Quotient and remainder of a cell key (< 2^40) by a grid extent (< 2^31),
@@ -244,19 +269,20 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *ld
 // tid, tid + N/8, ... so each load instruction reads consecutive points (the
 // network sorts any assignment; a key carries its own slot).
 template <int N>
-__device__ __forceinline__ void sort_tile_keys(const StreamDev &S, const float *in, int len, int tile0, int slot0,
+__device__ __forceinline__ void sort_tile_keys(const KeyGrid &G, gptr<const float> in, int len, int tile0, int slot0,
                                                uint64_t (&v)[8], uint64_t *lds, int tid) {
-    const int dim = S.dim;
+    const int dim = G.dim;
     float c[8][3];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int gi = min(tile0 + e * (N / 8) + tid, len - 1);
+#pragma unroll
         for (int a = 0; a < 3; a++) c[e][a] = a < dim ? in[(size_t)gi * dim + a] : 0.f;
     }
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int gi = tile0 + e * (N / 8) + tid;
-        v[e] = gi < len ? (cell_key_of(S, c[e]) << S.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
+        v[e] = gi < len ? (cell_key_g(G, c[e][0], c[e][1], c[e][2]) << G.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
     }
     bitonic_sort_regs<N>(v, lds, tid);
 }
@@ -270,7 +296,7 @@ __global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
     const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int tile0 = (g - ldc(a.pre_itile[s])) * kTile;
     uint64_t v[8];
-    sort_tile_keys<kTile>(S, io.in, S.W, tile0, 0, v, lds, threadIdx.x);
+    sort_tile_keys<kTile>(key_grid(S), gl(io.in), ldc(S.W), tile0, 0, v, lds, threadIdx.x);
     const int nvalid = min(kTile, S.W - tile0);
     const uint32_t m = slot_mask(S);
     const int dim = S.dim;
@@ -288,10 +314,6 @@ __global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
     }
 }
 
-__device__ __forceinline__ bool survives(const StreamDev &S, uint64_t key, int lo_slot) {
-    const int slot = (int)((uint32_t)key & slot_mask(S));
-    return slot < lo_slot || slot >= lo_slot + S.S;
-}
 
 // Survivors per window tile (the reference's stable_partition,
 // thrust_helper.cu:37-39, split into count + compact).
@@ -299,20 +321,25 @@ __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds
     const int s = find_stream_c(a.pre_wtile, a.n, g);
     const StreamDev &S = a.s[s];
     const int tile = g - ldc(a.pre_wtile[s]);
-    const int p0 = tile * kWTile, p1 = min(p0 + kWTile, S.W);
-    const int lo_slot = S.mut->u * S.S;
+    const int W = ldc(S.W), SS = ldc(S.S);
+    const gptr<const uint64_t> skey = gl((const uint64_t *)ldc(S.skey));
+    const uint32_t sm = (uint32_t)((1ull << ldc(S.slot_bits)) - 1ull);
+    const int p0 = tile * kWTile, p1 = min(p0 + kWTile, W);
+    const int lo_slot = ldc(ldc(S.mut)->u) * SS;
     uint64_t k[kWTile / kTileThreads];
 #pragma unroll
     for (int r = 0; r < kWTile / kTileThreads; r++) {
         const int p = p0 + threadIdx.x + r * kTileThreads;
-        k[r] = p < p1 ? S.skey[p] : 0ull;
+        k[r] = p < p1 ? skey[p] : 0ull;
     }
     int cnt = 0;
 #pragma unroll
-    for (int r = 0; r < kWTile / kTileThreads; r++)
-        cnt += (p0 + (int)threadIdx.x + r * kTileThreads < p1) && survives(S, k[r], lo_slot);
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
+        const int slot = (int)((uint32_t)k[r] & sm);
+        cnt += (p0 + (int)threadIdx.x + r * kTileThreads < p1) && (slot < lo_slot || slot >= lo_slot + SS);
+    }
     cnt = block_sum(cnt, lds4);
-    if (threadIdx.x == 0) S.tile_a[tile] = cnt;
+    if (threadIdx.x == 0) gl(ldc(S.tile_a))[tile] = cnt;
 }
 
 // Launch 1 of a slide: blocks [0, stride tiles) sort the stride by key (the
@@ -332,16 +359,19 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     const int s = find_stream_c(a.pre_stile, a.n, g);
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
+    const int SS = ldc(S.S);
     const int tile0 = (g - ldc(a.pre_stile[s])) * kTile;
     const int w = threadIdx.x >> 6;
     uint64_t v[8];
-    sort_tile_keys<kRun>(S, io.in, S.S, tile0 + w * kRun, S.mut->u * S.S, v, nullptr, lane_id());
+    sort_tile_keys<kRun>(key_grid(S), gl(io.in), SS, tile0 + w * kRun, ldc(ldc(S.mut)->u) * SS, v, nullptr,
+                         lane_id());
     // coalesced store through LDS
 #pragma unroll
     for (int e = 0; e < 8; e++) lds[threadIdx.x * 8 + e] = v[e];
     __syncthreads();
-    const int nvalid = min(kTile, S.S - tile0);
-    for (int i = threadIdx.x; i < nvalid; i += kTileThreads) S.bkey0[tile0 + i] = lds[i];
+    const int nvalid = min(kTile, SS - tile0);
+    const gptr<uint64_t> bkey0 = gl(ldc(S.bkey0));
+    for (int i = threadIdx.x; i < nvalid; i += kTileThreads) bkey0[tile0 + i] = lds[i];
 }
 
 // Ordered compaction of the survivors into the A arrays. Each tile's offset
@@ -351,36 +381,44 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
     const StreamDev &S = a.s[s];
     const int tile = g - ldc(a.pre_wtile[s]);
     const int p0 = tile * kWTile;
-    const int lo_slot = S.mut->u * S.S;
+    const int W = ldc(S.W), SS = ldc(S.S), dim = ldc(S.dim);
+    const uint32_t sm = (uint32_t)((1ull << ldc(S.slot_bits)) - 1ull);
+    const int lo_slot = ldc(ldc(S.mut)->u) * SS;
+    const gptr<const uint64_t> skey = gl((const uint64_t *)ldc(S.skey));
+    const gptr<const float> sx = gl((const float *)ldc(S.sx)), sy = gl((const float *)ldc(S.sy)),
+                            sz = gl((const float *)ldc(S.sz));
+    const gptr<const int32_t> tile_a = gl((const int32_t *)ldc(S.tile_a));
     constexpr int per = kWTile / kTileThreads;
-    const int dim = S.dim;
     // every load of the tile issued before the scan and the stores
     uint64_t key[per];
     float cx[per], cy[per], cz[per];
 #pragma unroll
     for (int r = 0; r < per; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
-        const int pp = p < S.W ? p : 0;
-        key[r] = S.skey[pp];
-        cx[r] = S.sx[pp];
-        cy[r] = dim > 1 ? S.sy[pp] : 0.f;
-        cz[r] = dim > 2 ? S.sz[pp] : 0.f;
+        const int pp = p < W ? p : 0;
+        key[r] = skey[pp];
+        cx[r] = sx[pp];
+        cy[r] = dim > 1 ? sy[pp] : 0.f;
+        cz[r] = dim > 2 ? sz[pp] : 0.f;
     }
     int part = 0;
-    for (int t = threadIdx.x; t < tile; t += kTileThreads) part += S.tile_a[t];
+    for (int t = threadIdx.x; t < tile; t += kTileThreads) part += tile_a[t];
     int run = block_sum(part, lds4);
+    const gptr<uint64_t> akey = gl(ldc(S.akey));
+    const gptr<float> ax = gl(ldc(S.ax)), ay = gl(ldc(S.ay)), az = gl(ldc(S.az));
 #pragma unroll
     for (int r = 0; r < per; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
-        const bool f = p < S.W && survives(S, key[r], lo_slot);
+        const int slot = (int)((uint32_t)key[r] & sm);
+        const bool f = p < W && (slot < lo_slot || slot >= lo_slot + SS);
         int tot;
         const int rank = block_rank_flags(f, &tot, lds4);
         if (f) {
             const int d = run + rank;
-            S.akey[d] = key[r];
-            S.ax[d] = cx[r];
-            if (dim > 1) S.ay[d] = cy[r];
-            if (dim > 2) S.az[d] = cz[r];
+            akey[d] = key[r];
+            ax[d] = cx[r];
+            if (dim > 1) ay[d] = cy[r];
+            if (dim > 2) az[d] = cz[r];
         }
         run += tot;
     }
