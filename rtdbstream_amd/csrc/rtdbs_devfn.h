@@ -46,9 +46,40 @@ __device__ __forceinline__ float sqd(int dim, float ax, float ay, float az, floa
     return d;
 }
 
+// v of lane (this lane ^ m), for m a power of two known at compile time: DPP
+// moves within 16-lane rows for m <= 8 (quad permutes, row shifts, a row
+// rotate) and the row-pair / half-wave swaps for 16 and 32 - no LDS round
+// trip, unlike __shfl_xor (ds_bpermute).
+__device__ __forceinline__ int xor_lane(int v, int m) {
+    const int lane = lane_id();
+    switch (m) {
+    case 1:
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    case 2:
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    case 4: {
+        const int up = __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, false);  // row_shl:4, from lane + 4
+        const int dn = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4, from lane - 4
+        return (lane & 4) ? dn : up;
+    }
+    case 8:
+        return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    case 16: {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? r[0] : r[1];
+    }
+    case 32: {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32) ? r[0] : r[1];
+    }
+    default:
+        return __shfl_xor(v, m);
+    }
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    for (int o = 32; o > 0; o >>= 1) v = min(v, xor_lane(v, o));
     return v;
 }
 

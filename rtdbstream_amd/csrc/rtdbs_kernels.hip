@@ -66,7 +66,7 @@ __device__ __forceinline__ uint32_t slot_mask(const StreamDev &S) {
 
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = 32; o > 0; o >>= 1) v += xor_lane(v, o);
     return v;
 }
 
@@ -212,8 +212,8 @@ __device__ __forceinline__ void divmod_key(uint64_t x, int64_t d, int64_t *q, in
 // merge-path merges of the window
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-    const int lo = __shfl_xor((int)(uint32_t)v, m);
-    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    const int lo = xor_lane((int)(uint32_t)v, m);
+    const int hi = xor_lane((int)(uint32_t)(v >> 32), m);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
@@ -1147,14 +1147,14 @@ __device__ __forceinline__ QueryLane<LPQ> query_lane(const BatchArgs &a, int vb)
 template <int LPQ>
 __device__ __forceinline__ int group_sum(int v) {
 #pragma unroll
-    for (int o = 1; o < LPQ; o <<= 1) v += __shfl_xor(v, o);
+    for (int o = 1; o < LPQ; o <<= 1) v += xor_lane(v, o);
     return v;
 }
 
 template <int LPQ>
 __device__ __forceinline__ int group_min(int v) {
 #pragma unroll
-    for (int o = 1; o < LPQ; o <<= 1) v = min(v, __shfl_xor(v, o));
+    for (int o = 1; o < LPQ; o <<= 1) v = min(v, xor_lane(v, o));
     return v;
 }
 
