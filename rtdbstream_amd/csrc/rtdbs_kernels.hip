@@ -47,6 +47,25 @@ __device__ __forceinline__ int find_stream(const T *pre, int n, long long g) {
     return lo;
 }
 
+// The stream of cell g for a wave that walks its cells in increasing order:
+// the largest s >= s0 with pre[s] <= g (pre[s0] <= g), by galloping from the
+// previous cell's stream - one probe while the stream stays the same -
+// instead of a full binary search per cell. Scalar loads (constant memory).
+__device__ __forceinline__ int find_stream_from(const int32_t *pre, int n, int g, int s0) {
+    int lo = s0, hi = s0 + 1, step = 1;
+    while (hi < n && ldc(pre[hi]) <= g) {
+        lo = hi;
+        step <<= 1;
+        hi = min(lo + step, n);
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ldc(pre[mid]) <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // The same over a per-batch prefix table in global memory, which no kernel
 // writes during a launch: scalar loads (the table stays in the scalar cache),
 // not a dependent vector load per step.
@@ -1339,11 +1358,12 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     ST st;
+    int s = 0;  // stream of the wave's current cell (cells rise, so streams do)
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
-        const CellCtx x = cell_of(a, gc);
-        const int s = uni(x.s);
+        s = uni(find_stream_from(a.cell_pre, a.n, gc, s));
+        const int c = gc - uni(ldc(a.cell_pre[s]));
         stats_stream(a, st, s);
-        count_cell(cell_view(a.s[s]), uni(x.c), st);
+        count_cell(cell_view(a.s[s]), c, st);
     }
     stats_flush(a, st);
 }
@@ -1455,11 +1475,12 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     ST st;
+    int s = 0;  // stream of the wave's current cell (cells rise, so streams do)
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
-        const CellCtx x = cell_of(a, gc);
-        const int s = uni(x.s);
+        s = uni(find_stream_from(a.cell_pre, a.n, gc, s));
+        const int c = gc - uni(ldc(a.cell_pre[s]));
         stats_stream(a, st, s);
-        output_cell(a, s, cell_view(a.s[s]), a.use_io1 ? a.io1 : a.io[s], uni(x.c), st);
+        output_cell(a, s, cell_view(a.s[s]), a.use_io1 ? a.io1 : a.io[s], c, st);
     }
     stats_flush(a, st);
 }
@@ -1618,11 +1639,12 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     const int nw = gridDim.x * kCellWaves;
     ST st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0};
+    int s = 0;  // stream of the wave's current cell (cells rise, so streams do)
     for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
-        const CellCtx x = cell_of(a, gc);
-        const int s = uni(x.s);
+        s = uni(find_stream_from(a.cell_pre, a.n, gc, s));
+        const int c = gc - uni(ldc(a.cell_pre[s]));
         stats_stream(a, st, s);
-        union_cell(a, s, cell_view(a.s[s]), uni(x.c), st, eb);
+        union_cell(a, s, cell_view(a.s[s]), c, st, eb);
     }
     edge_flush(a, eb);
     stats_flush(a, st);
