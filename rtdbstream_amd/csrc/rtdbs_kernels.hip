@@ -1492,6 +1492,45 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
 // adding up cell after cell.
 constexpr int kEdgeCap = 256;
 
+// One cell's header as the merge needs it: stream, cell within the stream,
+// first core position (kNoCore: none, or no cell), flags and position range.
+struct CellHead {
+    int s, c, fc, cf, cs, ce;
+};
+
+// Lane l's cell g = g0 + l * nw of a wave's grid-stride sequence: its stream
+// by a per-lane binary search over the batch's cell prefix (a small table
+// that stays in cache), then the header, all lanes' loads in one round. A
+// cell past `total` reads as a cell without cores.
+__device__ __forceinline__ CellHead cell_head(const BatchArgs &a, long long g, int total) {
+    CellHead h{0, 0, kNoCore, 0, 0, 0};
+    if (g >= total) return h;
+    const gptr<const int32_t> pre = gl((const int32_t *)a.cell_pre);
+    int lo = 0, hi = a.n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)pre[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    h.s = lo;
+    h.c = (int)(g - pre[lo]);
+    const gptr<const StreamDev> sd = gl(a.s) + lo;
+    const gptr<const int32_t> fcore = gl((const int32_t *)sd->fcore);
+    const gptr<const uint8_t> cflag = gl(sd->cflag);
+    const gptr<const int32_t> cstart = gl((const int32_t *)sd->cstart);
+    h.fc = fcore[h.c];
+    h.cf = cflag[h.c];
+    h.cs = cstart[h.c];
+    h.ce = cstart[h.c + 1];
+    return h;
+}
+
+// Lane j's header, wave-uniform.
+__device__ __forceinline__ CellHead lane_head(const CellHead &m, int j) {
+    return CellHead{lane_i(m.s, j), lane_i(m.c, j), lane_i(m.fc, j), lane_i(m.cf, j), lane_i(m.cs, j),
+                    lane_i(m.ce, j)};
+}
+
 struct Edge {
     int x, y;  // union-find elements (slots)
     int s;     // stream
@@ -1543,22 +1582,18 @@ __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
 template <class ST>
-__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, ST &st, EdgeBuf &eb) {
+__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const CellHead &h, ST &st, EdgeBuf &eb) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
-    // loads in rounds, as in count_cell: header and rows, queries, chunks
-    const int fcv = V.fcore[A], cfv = V.cflag[A], csv = V.cstart[A], cev = V.cstart[A + 1];
+    // the header comes screened (cell_heads); the rows load with the first
+    // queries, then each candidate chunk is a round of its own
+    const int A = h.c;
+    const bool safeA = h.cf & kCellSafe, denseA = h.cf & kCellDense;
+    const int cs = h.cs, nA = h.ce - h.cs;
     const uint64_t rw = row_word(V, A);
-    arrived(rw);
-    const int fcA = uni(fcv);
-    if (fcA == kNoCore) return;
-    const int cfA = uni(cfv);
-    const bool safeA = cfA & kCellSafe, denseA = cfA & kCellDense;
-    const int cs = uni(csv), nA = uni(cev) - cs;
-    const RowWalk w0 = row_walk(rw, safeA ? cs : cs + nA);
-    if (w0.rest == 0ull) return;
-    const uint64_t rk = V.skey[fcA];  // the representative's slot, with the first queries
+    const uint64_t rk = V.skey[h.fc];  // the representative's slot
+    RowWalk w0;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1571,6 +1606,11 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
         arrived(qy);
         arrived(qz);
         arrived(qk);
+        if (q0 == 0) {
+            arrived(rw);
+            w0 = row_walk(rw, safeA ? cs : cs + nA);
+            if (w0.rest == 0ull) return;
+        }
         const bool qcore = lane < nq && (denseA || (qf & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
@@ -1607,17 +1647,18 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, int A, 
                 const uint64_t below = starts & ((2ull << lane) - 1ull);
                 const uint64_t from = ~((1ull << (63 - __clzll((long long)below))) - 1ull);
                 const uint64_t run = upto & from;  // this lane's run
-                bool touched = false;
+                // touched candidates as one wave mask: the loop carries a scalar,
+                // not a lane predicate
+                uint64_t t = 0;
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
                     st.add(kStatMergeTests, chunk);
-                    // bitwise: every lane evaluates the test, no branch per query
-                    touched |= bcore & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
-                    const uint64_t t = __ballot(touched);
-                    if (__ballot(!bcore | ((t & run) != 0ull)) == ~0ull) break;  // every candidate run settled
+                    // every lane evaluates the test, no branch per query
+                    t |= __ballot(sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2) & bm;
+                    if ((__ballot((t & run) != 0ull) | ~bm) == ~0ull) break;  // every candidate run settled
                 }
-                unite_once_per_rep(a, eb, s, touched, rep, repA, st);
+                unite_once_per_rep(a, eb, s, (t >> lane) & 1ull, rep, repA, st);
             } else {
                 while (m) {
                     const int qi = __ffsll((unsigned long long)m) - 1;
@@ -1639,12 +1680,15 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
     const int nw = gridDim.x * kCellWaves;
     ST st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0};
-    int s = 0;  // stream of the wave's current cell (cells rise, so streams do)
-    for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
-        s = uni(find_stream_from(a.cell_pre, a.n, gc, s));
-        const int c = gc - uni(ldc(a.cell_pre[s]));
-        stats_stream(a, st, s);
-        union_cell(a, s, cell_view(a.s[s]), c, st, eb);
+    const int w = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6));
+    for (long long g0 = w; g0 < total; g0 += 64ll * nw) {
+        const CellHead mine = cell_head(a, g0 + (long long)lane_id() * nw, total);
+        for (uint64_t m = __ballot(mine.fc != kNoCore); m; m &= m - 1ull) {
+            const int j = __ffsll((unsigned long long)m) - 1;
+            const CellHead h = lane_head(mine, j);
+            stats_stream(a, st, h.s);
+            union_cell(a, h.s, cell_view(a.s[h.s]), h, st, eb);
+        }
     }
     edge_flush(a, eb);
     stats_flush(a, st);
