@@ -99,8 +99,8 @@ __device__ __forceinline__ int uf_load(const int32_t *p) { return *p; }
 // the displaced parent `old`, so no link is lost and b strictly decreases.
 // (A compare-and-swap hook fails and retries whenever another lane hooked the
 // same root first, which serialises every edge of a large cluster on its
-// root.) Path halving also lowers entries with atomicMin, so every entry only
-// ever decreases and always points at an ancestor.
+// root.) Path halving and compression also lower entries with atomicMin, so
+// every entry only ever decreases and always points at an ancestor.
 __device__ __forceinline__ int uf_find(int32_t *parent, int x) {
     int p = uf_load(parent + x);
     while (p != x) {
@@ -112,24 +112,31 @@ __device__ __forceinline__ int uf_find(int32_t *parent, int x) {
     return x;
 }
 
-// The roots of a and b, walked together (uf_find's path halving on both
-// paths) so that the loads of both walks are in flight at once.
+// The roots of a and b, walked together so that the loads of both walks are
+// in flight at once. The walks only load; afterwards each starting entry is
+// pointed at the root found (one atomicMin each). Atomics inside a walk would
+// stall it: the vector memory counter also counts atomics, in issue order, so
+// the wait for the next step's load would include the atomic issued before
+// it. The starting entries are the elements the merge unites (cell
+// representatives, core slots), so their next walks take one step.
 __device__ __forceinline__ void uf_find2(int32_t *parent, int &a, int &b) {
+    const int a0 = a, b0 = b;
     int pa = uf_load(parent + a), pb = uf_load(parent + b);
+    const int pa0 = pa, pb0 = pb;
     while (pa != a || pb != b) {
         const int ga = pa != a ? uf_load(parent + pa) : a;
         const int gb = pb != b ? uf_load(parent + pb) : b;
         if (pa != a) {
-            if (ga != pa) atomicMin(parent + a, ga);
             a = pa;
             pa = ga;
         }
         if (pb != b) {
-            if (gb != pb) atomicMin(parent + b, gb);
             b = pb;
             pb = gb;
         }
     }
+    if (a != pa0) atomicMin(parent + a0, a);
+    if (b != pb0) atomicMin(parent + b0, b);
 }
 
 __device__ __forceinline__ void uf_unite(int32_t *parent, int a, int b) {
