@@ -1500,28 +1500,48 @@ struct CellHead {
 
 // Lane l's cell g = g0 + l * nw of a wave's grid-stride sequence: its stream
 // by a per-lane binary search over the batch's cell prefix (a small table
-// that stays in cache), then the header, all lanes' loads in one round. A
-// cell past `total` reads as a cell without cores.
+// that stays in cache), then the header, all lanes' loads in one round. When
+// every lane's cell is in one stream (always for a single stream) the array
+// bases come from scalar loads of that stream's descriptor, otherwise from
+// per-lane loads of each lane's descriptor. A cell past `total` reads as a
+// cell without cores. Called by the whole wave.
 __device__ __forceinline__ CellHead cell_head(const BatchArgs &a, long long g, int total) {
     CellHead h{0, 0, kNoCore, 0, 0, 0};
-    if (g >= total) return h;
+    const bool valid = g < total;
     const gptr<const int32_t> pre = gl((const int32_t *)a.cell_pre);
-    int lo = 0, hi = a.n;
-    while (hi - lo > 1) {
+    int lo = 0, hi = a.n, plo = 0;  // pre[0] = 0
+    while (valid && hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if ((long long)pre[mid] <= g) lo = mid;
-        else hi = mid;
+        const int v = pre[mid];
+        if ((long long)v <= g) {
+            lo = mid;
+            plo = v;
+        } else {
+            hi = mid;
+        }
     }
     h.s = lo;
-    h.c = (int)(g - pre[lo]);
-    const gptr<const StreamDev> sd = gl(a.s) + lo;
-    const gptr<const int32_t> fcore = gl((const int32_t *)sd->fcore);
-    const gptr<const uint8_t> cflag = gl(sd->cflag);
-    const gptr<const int32_t> cstart = gl((const int32_t *)sd->cstart);
-    h.fc = fcore[h.c];
-    h.cf = cflag[h.c];
-    h.cs = cstart[h.c];
-    h.ce = cstart[h.c + 1];
+    h.c = valid ? (int)(g - plo) : 0;
+    const int s0 = uni(h.s);  // lane 0's cell is valid
+    gptr<const int32_t> fcore, cstart;
+    gptr<const uint8_t> cflag;
+    if (__ballot(valid && h.s != s0) == 0ull) {
+        const StreamDev &S = a.s[s0];
+        fcore = gl((const int32_t *)ldc(S.fcore));
+        cflag = gl((const uint8_t *)ldc(S.cflag));
+        cstart = gl((const int32_t *)ldc(S.cstart));
+    } else {
+        const gptr<const StreamDev> sd = gl(a.s) + h.s;
+        fcore = gl((const int32_t *)sd->fcore);
+        cflag = gl((const uint8_t *)sd->cflag);
+        cstart = gl((const int32_t *)sd->cstart);
+    }
+    if (valid) {
+        h.fc = fcore[h.c];
+        h.cf = cflag[h.c];
+        h.cs = cstart[h.c];
+        h.ce = cstart[h.c + 1];
+    }
     return h;
 }
 
