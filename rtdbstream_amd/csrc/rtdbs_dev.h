@@ -119,6 +119,7 @@ struct LaunchShape {
     int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels
     int32_t cellpar;         // 1: cell-parallel count/border, 0: point-parallel
     int32_t union_cells;     // point-parallel path: 1 cell-level merge kernels, 0 the cell-parallel merge
+    int32_t rank_split;      // 1: stride rank with 64-key blocks (small strides), 0: 256-key blocks
 };
 
 enum Phase {

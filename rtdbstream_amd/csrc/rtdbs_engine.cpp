@@ -577,6 +577,8 @@ int refresh_batch(rtdbs_batch *b) {
     b->shape.stiles = pre[0 * (n + 1) + n];
     b->shape.wtiles = pre[1 * (n + 1) + n];
     b->shape.chunks = pre[2 * (n + 1) + n];
+    // stride rank: 64-key blocks when 256-key blocks would not cover the device
+    b->shape.rank_split = (long long)b->shape.stiles * (kTile / kTileThreads) < device_cus() ? 1 : 0;
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));

@@ -443,35 +443,46 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
     }
 }
 
-// Launch 2 of a slide: kRankParts blocks per stride tile place each key at
-// its rank in the whole stride - its index in its sorted run plus, for every
-// other run, the number of that run's keys below it - and write keys and
+// Launch 2 of a slide: blocks of each stride tile place each key at its rank
+// in the whole stride - its index in its sorted run plus, for every other
+// run, the number of that run's keys below it - and write keys and
 // coordinates to the B1 arrays: the stride sorted by (cell, slot). The other
-// runs are staged in LDS kStageRuns at a time and the binary searches of all
-// staged runs advance together; one key per thread.
-constexpr int kRankParts = kTile / kTileThreads;
+// runs are staged in LDS kStageRuns at a time (the next ones load while the
+// current ones are searched). With SPLIT == 1 a block ranks 256 keys, one per
+// thread, and every thread searches every staged run. With SPLIT == 4 a
+// block ranks 64 keys and its four waves split each stage's runs between
+// them, summing their partial ranks in LDS at the end: four times the blocks
+// and a quarter of the searches per thread, for strides too small to fill the
+// device with 256-key blocks.
 constexpr int kStageRuns = 8;  // 32 KB of LDS
 
+template <int SPLIT>
 __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
+    constexpr int kWaves = kTileThreads / 64;
+    constexpr int kKeys = kTileThreads / SPLIT;  // keys per block, all in one run (kKeys divides kRun)
+    constexpr int kParts = kTile / kKeys;        // blocks per stride tile
+    constexpr int kMine = kStageRuns / SPLIT;    // staged runs each thread searches
     __shared__ uint64_t lds[kStageRuns * kRun];
+    __shared__ int part[SPLIT > 1 ? kWaves : 1][64];
     const int g = blockIdx.x;
-    const int gb = g / kRankParts, part = g % kRankParts;
+    const int gb = g / kParts;
     const int s = find_stream_c(a.pre_stile, a.n, gb);
     const StreamDev &S = a.s[s];
     // the stream's fields as constant memory and its arrays as global memory:
     // global loads wait on their own counter, so the next stage's keys load
     // while the LDS searches of this stage run
     const int SS = ldc(S.S), dim = ldc(S.dim);
+    const int i0 = (gb - ldc(a.pre_stile[s])) * kTile + (g % kParts) * kKeys;
+    if (i0 >= SS) return;  // the whole block: past the stride's end
     const gptr<const uint64_t> bkey0 = gl((const uint64_t *)ldc(S.bkey0));
-    const int i = (gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads + threadIdx.x;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int sigma = wv % SPLIT;  // which of each stage's runs this wave searches
+    const int i = i0 + (SPLIT > 1 ? lane : (int)threadIdx.x);
     const bool valid = i < SS;
     const uint64_t key = valid ? bkey0[i] : ~0ull;
-    const int myrun = valid ? i / kRun : -1, nr = (SS + kRun - 1) / kRun;
-    int rank = valid ? i - myrun * kRun : 0;
-    // the runs other than this block's own ones are staged; a block's keys
-    // share at most one run (kTileThreads divides kRun)
-    const int blockrun = ((gb - ldc(a.pre_stile[s])) * kTile + part * kTileThreads) / kRun;
+    const int myrun = i0 / kRun, nr = (SS + kRun - 1) / kRun;
     constexpr int per = kStageRuns * kRun / kTileThreads;
+    auto run_of = [&](int o) { return o < myrun ? o : o + 1; };  // the o-th run other than the block's
     // keys of the staged runs o0 .. o0 + kStageRuns - 1, ~0 past their ends
     auto load_stage = [&](int o0, uint64_t(&st)[per]) {
 #pragma unroll
@@ -479,7 +490,7 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
             const int slot = threadIdx.x + e * kTileThreads;  // [0, kStageRuns * kRun)
             const int q = slot / kRun, j = slot % kRun;
             const int o = o0 + q;
-            const int r = o < blockrun ? o : o + 1;  // the o-th run other than the block's
+            const int r = run_of(o);
             st[e] = (o < nr - 1 && j < min(kRun, SS - r * kRun)) ? bkey0[r * kRun + j] : ~0ull;
         }
     };
@@ -487,43 +498,56 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
     if (nr > 1) load_stage(0, cur);
     // the point's coordinates are needed only at the end: loaded now
     const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int gi = valid ? (int)((uint32_t)key & slot_mask(S)) - ldc(ldc(S.mut)->u) * SS : 0;
-    const gptr<const float> p = gl(io.in) + (size_t)gi * dim;
-    const float px = p[0], py = dim > 1 ? p[1] : 0.f, pz = dim > 2 ? p[2] : 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (sigma == 0) {
+        const int gi = valid ? (int)((uint32_t)key & slot_mask(S)) - ldc(ldc(S.mut)->u) * SS : 0;
+        const gptr<const float> p = gl(io.in) + (size_t)gi * dim;
+        px = p[0];
+        py = dim > 1 ? p[1] : 0.f;
+        pz = dim > 2 ? p[2] : 0.f;
+    }
+    int rank = 0;
     for (int o0 = 0; o0 < nr - 1; o0 += kStageRuns) {
         const bool more = o0 + kStageRuns < nr - 1;
         if (more) load_stage(o0 + kStageRuns, nxt);
-        int len[kStageRuns];
+        int len[kMine];
 #pragma unroll
-        for (int q = 0; q < kStageRuns; q++) {
-            const int o = o0 + q;
-            const int r = o < blockrun ? o : o + 1;
-            len[q] = o < nr - 1 ? min(kRun, SS - r * kRun) : 0;
+        for (int q = 0; q < kMine; q++) {
+            const int o = o0 + q * SPLIT + sigma;
+            len[q] = o < nr - 1 ? min(kRun, SS - run_of(o) * kRun) : 0;
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < per; e++) lds[threadIdx.x + e * kTileThreads] = cur[e];
         __syncthreads();
-        int pos[kStageRuns];
+        int pos[kMine];
 #pragma unroll
-        for (int q = 0; q < kStageRuns; q++) pos[q] = 0;
+        for (int q = 0; q < kMine; q++) pos[q] = 0;
 #pragma unroll
         for (int step = kRun; step > 0; step >>= 1) {
 #pragma unroll
-            for (int q = 0; q < kStageRuns; q++) {
+            for (int q = 0; q < kMine; q++) {
                 const int probe = pos[q] + step;
-                const uint64_t v = lds[q * kRun + min(probe, kRun) - 1];
+                const uint64_t v = lds[(q * SPLIT + sigma) * kRun + min(probe, kRun) - 1];
                 if (probe <= len[q] && v < key) pos[q] = probe;
             }
         }
 #pragma unroll
-        for (int q = 0; q < kStageRuns; q++) rank += pos[q];
+        for (int q = 0; q < kMine; q++) rank += pos[q];
         if (more) {
 #pragma unroll
             for (int e = 0; e < per; e++) cur[e] = nxt[e];
         }
     }
+    if (SPLIT > 1) {
+        part[wv][lane] = rank;
+        __syncthreads();
+        if (sigma != 0) return;
+#pragma unroll
+        for (int w = 1; w < SPLIT; w++) rank += part[w][lane];
+    }
     if (!valid) return;
+    rank += i - myrun * kRun;
     gl(S.bkey1)[rank] = key;
     gl(S.bx1)[rank] = px;
     if (dim > 1) gl(S.by1)[rank] = py;
@@ -2408,7 +2432,8 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_sort_expire, sh.stiles + sh.wtiles, kTileThreads, a);
     PH_END(kPhStrideSort);
     PH_BEGIN(kPhMerge);
-    RTDBS_LAUNCH(k_rank, sh.stiles * kRankParts, kTileThreads, a);
+    if (sh.rank_split) RTDBS_LAUNCH(k_rank<4>, sh.stiles * (kTile / 64), kTileThreads, a);
+    else RTDBS_LAUNCH(k_rank<1>, sh.stiles * (kTile / kTileThreads), kTileThreads, a);
     RTDBS_LAUNCH(k_compact, sh.wtiles, kTileThreads, a);
     PH_END(kPhMerge);
     PH_BEGIN(kPhMergeFinal);
