@@ -236,6 +236,28 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// Compare-exchange of lanes l and l ^ m (m a power of two below 64): the
+// smaller key where keep_min, else the larger. For m = 16 and 32 the row-pair
+// and half-wave swaps of both copies of a word leave the lower partner's
+// value in one register and the upper partner's in the other, in every lane,
+// so no select picks the partner out first.
+__device__ __forceinline__ uint64_t lane_minmax(uint64_t v, int m, bool keep_min) {
+    uint64_t x, y;
+    if (m == 16 || m == 32) {
+        const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+        const auto rl = m == 32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                                : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto rh = m == 32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                                : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        x = ((uint64_t)(uint32_t)rh[0] << 32) | (uint32_t)rl[0];  // lane l & ~m
+        y = ((uint64_t)(uint32_t)rh[1] << 32) | (uint32_t)rl[1];  // lane l | m
+    } else {
+        x = v;
+        y = shfl_xor_u64(v, m);
+    }
+    return ((y < x) == keep_min) ? y : x;
+}
+
 // Sorts N keys ascending, 8 per thread: thread t (of N/8) holds positions
 // 8t..8t+7 in v. Bitonic network: exchanges between a thread's own elements
 // stay in registers, exchanges within a wave use lane shuffles, and only the
@@ -258,14 +280,11 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *ld
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
                         const uint64_t o = lds[(tid ^ m) * 8 + e];
-                        v[e] = keep_min ? (o < v[e] ? o : v[e]) : (o > v[e] ? o : v[e]);
+                        v[e] = ((o < v[e]) == keep_min) ? o : v[e];
                     }
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const uint64_t o = shfl_xor_u64(v[e], m);
-                        v[e] = keep_min ? (o < v[e] ? o : v[e]) : (o > v[e] ? o : v[e]);
-                    }
+                    for (int e = 0; e < 8; e++) v[e] = lane_minmax(v[e], m, keep_min);
                 }
             } else {
 #pragma unroll
