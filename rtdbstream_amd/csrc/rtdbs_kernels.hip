@@ -930,7 +930,11 @@ __device__ __forceinline__ int rows_groups(int dim, int k, int R) { return dim =
 
 // Item g of tile t: the rows with offset o0 = g - k (3-D), or row g; the
 // 64 cells of the tile one per lane. The item holding the centre row also
-// records each cell's class. lk/lc: this wave's LDS.
+// records each cell's class. lk/lc: this wave's LDS. NS >= the item's row
+// count (2k + 1 <= 7 in 3-D, 1 otherwise): the lower bounds of all its rows
+// in the staged bracket are searched in lockstep, one LDS round per step for
+// all of them.
+template <int NS>
 __device__ __forceinline__ void rows_tile(const RowsView &V, int t, int g, uint64_t *lk, int32_t *lc) {
     const int lane = lane_id();
     const int c = t * kRowTile + lane;
@@ -979,23 +983,50 @@ __device__ __forceinline__ void rows_tile(const RowsView &V, int t, int g, uint6
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    for (int j = 0; j < nsub; j++) {
+    // key bounds of each row: the target column's keys at zlo..zhi
+    uint64_t lo[NS], hi1[NS];
+    bool ok[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
         const int o1 = o1lo + j;
-        bool ok = cv;
-        if (V.dim == 3) ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0 && i1 + o1 >= 0 && i1 + o1 < V.n1;
-        else if (V.dim == 2) ok = ok && i0 + o0 >= 0 && i0 + o0 < V.n0;
-        // key bounds: the target column's keys at zlo..zhi
+        bool okj = cv && j < nsub;
+        if (V.dim == 3) okj = okj && i0 + o0 >= 0 && i0 + o0 < V.n0 && i1 + o1 >= 0 && i1 + o1 < V.n1;
+        else if (V.dim == 2) okj = okj && i0 + o0 >= 0 && i0 + o0 < V.n0;
         const int64_t colbase = (int64_t)ck - zc + ((int64_t)o0 * V.n1 + (V.dim == 3 ? o1 : 0)) * V.n2;
-        const uint64_t lo = (uint64_t)(colbase + (zc - k < 0 ? 0 : zc - k));
-        const uint64_t hi = (uint64_t)(colbase + (zc + k >= zn ? zn - 1 : zc + k));
+        lo[j] = (uint64_t)(colbase + (zc - k < 0 ? 0 : zc - k));
+        hi1[j] = (uint64_t)(colbase + (zc + k >= zn ? zn - 1 : zc + k)) + 1ull;
+        ok[j] = okj;
+    }
+    int f[NS], e[NS];  // first and end positions of each row's cells in the bracket
+    if (staged) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) f[j] = e[j] = 0;
+        for (int n = B; n > 1;) {
+            const int half = n >> 1;
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                if (lk[f[j] + half] < lo[j]) f[j] += half;
+                if (lk[e[j] + half] < hi1[j]) e[j] += half;
+            }
+            n -= half;
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            f[j] += lk[f[j]] < lo[j] ? 1 : 0;
+            e[j] += lk[e[j]] < hi1[j] ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        if (j >= nsub) break;
         uint64_t row = 0ull;
-        if (ok && B > 0) {
+        if (ok[j] && B > 0) {
             if (staged) {
-                const int f = lower_index(lk, B, lo), e = lower_index(lk, B, hi + 1);
-                if (f < e) row = (uint64_t)(uint32_t)lc[f] | ((uint64_t)(uint32_t)lc[e] << 32);
+                if (f[j] < e[j]) row = (uint64_t)(uint32_t)lc[f[j]] | ((uint64_t)(uint32_t)lc[e[j]] << 32);
             } else {
-                const int f = b0 + lower_index(V.ckey + b0, B, lo), e = b0 + lower_index(V.ckey + b0, B, hi + 1);
-                if (f < e) row = (uint64_t)(uint32_t)V.cstart[f] | ((uint64_t)(uint32_t)V.cstart[e] << 32);
+                const int fg = b0 + lower_index(V.ckey + b0, B, lo[j]);
+                const int eg = b0 + lower_index(V.ckey + b0, B, hi1[j]);
+                if (fg < eg) row = (uint64_t)(uint32_t)V.cstart[fg] | ((uint64_t)(uint32_t)V.cstart[eg] << 32);
             }
         }
         // stencil row number, centre row first
@@ -1042,7 +1073,8 @@ __global__ __launch_bounds__(kTileThreads) void k_rows(BatchArgs a) {
         const int local = uni((int)(wi - pre[s]));
         const int G = rows_groups(V.dim, V.k, V.R);
         const int t = local / G;
-        rows_tile(V, t, local - t * G, lkey[wv], lcs[wv]);
+        if (V.dim == 3) rows_tile<7>(V, t, local - t * G, lkey[wv], lcs[wv]);
+        else rows_tile<1>(V, t, local - t * G, lkey[wv], lcs[wv]);
     }
 }
 
