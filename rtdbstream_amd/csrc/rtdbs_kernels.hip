@@ -462,7 +462,7 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
     }
 }
 
-// Launch 2 of a slide: blocks of each stride tile place each key at its rank
+// Launch 2 of a slide: blocks [0, rank_blocks) place each stride key at its rank
 // in the whole stride - its index in its sorted run plus, for every other
 // run, the number of that run's keys below it - and write keys and
 // coordinates to the B1 arrays: the stride sorted by (cell, slot). The other
@@ -472,17 +472,22 @@ __device__ __forceinline__ void expire_compact(const BatchArgs &a, int g, int *l
 // block ranks 64 keys and its four waves split each stage's runs between
 // them, summing their partial ranks in LDS at the end: four times the blocks
 // and a quarter of the searches per thread, for strides too small to fill the
-// device with 256-key blocks.
+// device with 256-key blocks. The blocks after them compact the window's
+// survivors (expire_compact), which depends on the same launch before.
 constexpr int kStageRuns = 8;  // 32 KB of LDS
 
 template <int SPLIT>
-__global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
+__global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a, int rank_blocks) {
     constexpr int kWaves = kTileThreads / 64;
     constexpr int kKeys = kTileThreads / SPLIT;  // keys per block, all in one run (kKeys divides kRun)
     constexpr int kParts = kTile / kKeys;        // blocks per stride tile
     constexpr int kMine = kStageRuns / SPLIT;    // staged runs each thread searches
     __shared__ uint64_t lds[kStageRuns * kRun];
     __shared__ int part[SPLIT > 1 ? kWaves : 1][64];
+    if ((int)blockIdx.x >= rank_blocks) {  // the survivor compaction shares the launch
+        expire_compact(a, blockIdx.x - rank_blocks, (int *)part);
+        return;
+    }
     const int g = blockIdx.x;
     const int gb = g / kParts;
     const int s = find_stream_c(a.pre_stile, a.n, gb);
@@ -573,11 +578,6 @@ __global__ __launch_bounds__(kTileThreads) void k_rank(BatchArgs a) {
     if (dim > 2) gl(S.bz1)[rank] = pz;
 }
 
-// Launch 3: ordered compaction of the window's survivors.
-__global__ __launch_bounds__(kTileThreads) void k_compact(BatchArgs a) {
-    __shared__ int lds4[4];
-    expire_compact(a, blockIdx.x, lds4);
-}
 
 // Number of elements of A among the first d of merge(A, B); keys are unique.
 // P(i) = A[i] < B[d-1-i] holds exactly for i below the answer. One wave
@@ -2483,9 +2483,13 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     RTDBS_LAUNCH(k_sort_expire, sh.stiles + sh.wtiles, kTileThreads, a);
     PH_END(kPhStrideSort);
     PH_BEGIN(kPhMerge);
-    if (sh.rank_split) RTDBS_LAUNCH(k_rank<4>, sh.stiles * (kTile / 64), kTileThreads, a);
-    else RTDBS_LAUNCH(k_rank<1>, sh.stiles * (kTile / kTileThreads), kTileThreads, a);
-    RTDBS_LAUNCH(k_compact, sh.wtiles, kTileThreads, a);
+    if (sh.rank_split) {
+        const int rb = sh.stiles * (kTile / 64);
+        RTDBS_LAUNCH(k_rank<4>, rb + sh.wtiles, kTileThreads, a, rb);
+    } else {
+        const int rb = sh.stiles * (kTile / kTileThreads);
+        RTDBS_LAUNCH(k_rank<1>, rb + sh.wtiles, kTileThreads, a, rb);
+    }
     PH_END(kPhMerge);
     PH_BEGIN(kPhMergeFinal);
     RTDBS_LAUNCH(k_merge_final, sh.wtiles, kTileThreads, a);
