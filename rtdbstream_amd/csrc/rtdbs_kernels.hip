@@ -258,39 +258,41 @@ __device__ __forceinline__ uint64_t lane_minmax(uint64_t v, int m, bool keep_min
     return ((y < x) == keep_min) ? y : x;
 }
 
-// Sorts N keys ascending, 8 per thread: thread t (of N/8) holds positions
-// 8t..8t+7 in v. Bitonic network: exchanges between a thread's own elements
-// stay in registers, exchanges within a wave use lane shuffles, and only the
-// stages whose partners sit in another wave (N = 2048: three) go through LDS
-// (`lds`, N keys) with block barriers; N = 512 is one wave and needs none.
-template <int N>
-__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *lds, int tid) {
+// Sorts N keys ascending, E per thread: thread t (of N/E) holds positions
+// Et..Et+E-1 in v. Bitonic network: exchanges between a thread's own elements
+// stay in registers, exchanges within a wave use lane exchanges, and only the
+// stages whose partners sit in another wave (j >= 64E) go through LDS (`lds`,
+// N keys) with block barriers: none for N = 64E (one wave), one stage for a
+// 512-key run over two waves (E = 4), three for N = 2048 over four waves
+// (E = 8). Every thread of the block takes part in the same stages.
+template <int N, int E>
+__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[E], uint64_t *lds, int tid) {
 #pragma unroll
     for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 8) {
-                const int m = j >> 3;  // partner thread = tid ^ m, same element index
-                const bool keep_min = ((tid & m) == 0) == (((tid * 8) & k) == 0);
-                if (j >= 512) {
+            if (j >= E) {
+                const int m = j / E;  // partner thread = tid ^ m, same element index
+                const bool keep_min = ((tid & m) == 0) == (((tid * E) & k) == 0);
+                if (j >= 64 * E) {
                     __syncthreads();
 #pragma unroll
-                    for (int e = 0; e < 8; e++) lds[tid * 8 + e] = v[e];
+                    for (int e = 0; e < E; e++) lds[tid * E + e] = v[e];
                     __syncthreads();
 #pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const uint64_t o = lds[(tid ^ m) * 8 + e];
+                    for (int e = 0; e < E; e++) {
+                        const uint64_t o = lds[(tid ^ m) * E + e];
                         v[e] = ((o < v[e]) == keep_min) ? o : v[e];
                     }
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 8; e++) v[e] = lane_minmax(v[e], m, keep_min);
+                    for (int e = 0; e < E; e++) v[e] = lane_minmax(v[e], m, keep_min);
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
+                for (int e = 0; e < E; e++) {
                     if (e & j) continue;
-                    const bool up = (((tid * 8) + e) & k) == 0;
+                    const bool up = (((tid * E) + e) & k) == 0;
                     const uint64_t x = v[e], y = v[e | j];
                     const bool sw = (x > y) == up;
                     v[e] = sw ? y : x;
@@ -302,27 +304,27 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[8], uint64_t *ld
 }
 
 // Loads the packed keys (cell << slot_bits | slot) of points [tile0, tile0 +
-// N) of `in` (len points, slots from slot0) into v, 8 per thread (tid < N/8),
+// N) of `in` (len points, slots from slot0) into v, E per thread (tid < N/E),
 // and sorts them; padding keys are ~0 and sort last. Thread tid takes points
-// tid, tid + N/8, ... so each load instruction reads consecutive points (the
+// tid, tid + N/E, ... so each load instruction reads consecutive points (the
 // network sorts any assignment; a key carries its own slot).
-template <int N>
+template <int N, int E>
 __device__ __forceinline__ void sort_tile_keys(const KeyGrid &G, gptr<const float> in, int len, int tile0, int slot0,
-                                               uint64_t (&v)[8], uint64_t *lds, int tid) {
+                                               uint64_t (&v)[E], uint64_t *lds, int tid) {
     const int dim = G.dim;
-    float c[8][3];
+    float c[E][3];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int gi = min(tile0 + e * (N / 8) + tid, len - 1);
+    for (int e = 0; e < E; e++) {
+        const int gi = min(tile0 + e * (N / E) + tid, len - 1);
 #pragma unroll
         for (int a = 0; a < 3; a++) c[e][a] = a < dim ? in[(size_t)gi * dim + a] : 0.f;
     }
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int gi = tile0 + e * (N / 8) + tid;
+    for (int e = 0; e < E; e++) {
+        const int gi = tile0 + e * (N / E) + tid;
         v[e] = gi < len ? (cell_key_g(G, c[e][0], c[e][1], c[e][2]) << G.slot_bits) | (uint64_t)(slot0 + gi) : ~0ull;
     }
-    bitonic_sort_regs<N>(v, lds, tid);
+    bitonic_sort_regs<N, E>(v, lds, tid);
 }
 
 // First window: keys and coordinates of each sorted tile (init ping-pong A).
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(kTileThreads) void k_init_tiles(BatchArgs a) {
     const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int tile0 = (g - ldc(a.pre_itile[s])) * kTile;
     uint64_t v[8];
-    sort_tile_keys<kTile>(key_grid(S), gl(io.in), ldc(S.W), tile0, 0, v, lds, threadIdx.x);
+    sort_tile_keys<kTile, 8>(key_grid(S), gl(io.in), ldc(S.W), tile0, 0, v, lds, threadIdx.x);
     const int nvalid = min(kTile, S.W - tile0);
     const uint32_t m = slot_mask(S);
     const int dim = S.dim;
@@ -380,34 +382,43 @@ __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds
     if (threadIdx.x == 0) gl(ldc(S.tile_a))[tile] = cnt;
 }
 
-// Launch 1 of a slide: blocks [0, stride tiles) sort the stride by key (the
-// reference's compute_cell_id + sortByCellIdAndOrder, aabb.cu:589-611,
-// thrust_helper.cu:21-29) in runs of kRun keys, one run per wave, into bkey0;
-// the remaining blocks count the survivors of each window tile.
-constexpr int kRun = 512;  // keys per sorted run of the stride (one wave)
+// Launch 1 of a slide: the first kSortParts blocks of each stride tile sort
+// the stride by key (the reference's compute_cell_id + sortByCellIdAndOrder,
+// aabb.cu:589-611, thrust_helper.cu:21-29) in runs of kRun keys into bkey0,
+// two waves per run (kSortE keys per thread: half the network's instructions
+// per wave of a one-wave run, for one LDS stage); the remaining blocks count
+// the survivors of each window tile.
+constexpr int kRun = 512;                                // keys per sorted run of the stride
+constexpr int kSortE = 4;                                // keys per thread
+constexpr int kSortRunThreads = kRun / kSortE;           // 128: two waves per run
+constexpr int kSortBlockKeys = kTileThreads * kSortE;    // 1024: two runs per block
+constexpr int kSortParts = kTile / kSortBlockKeys;       // blocks per stride tile
 
 __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
     __shared__ uint64_t lds[kTile];
     const int g = blockIdx.x;
-    const int nst = ldc(a.pre_stile[a.n]);
-    if (g >= nst) {
-        expire_count(a, g - nst, (int *)lds);
+    const int nsb = ldc(a.pre_stile[a.n]) * kSortParts;
+    if (g >= nsb) {
+        expire_count(a, g - nsb, (int *)lds);
         return;
     }
-    const int s = find_stream_c(a.pre_stile, a.n, g);
+    const int gt = g / kSortParts;
+    const int s = find_stream_c(a.pre_stile, a.n, gt);
     const StreamDev &S = a.s[s];
     const IO io = a.use_io1 ? a.io1 : a.io[s];
     const int SS = ldc(S.S);
-    const int tile0 = (g - ldc(a.pre_stile[s])) * kTile;
-    const int w = threadIdx.x >> 6;
-    uint64_t v[8];
-    sort_tile_keys<kRun>(key_grid(S), gl(io.in), SS, tile0 + w * kRun, ldc(ldc(S.mut)->u) * SS, v, nullptr,
-                         lane_id());
-    // coalesced store through LDS
-#pragma unroll
-    for (int e = 0; e < 8; e++) lds[threadIdx.x * 8 + e] = v[e];
+    const int tile0 = (gt - ldc(a.pre_stile[s])) * kTile + (g % kSortParts) * kSortBlockKeys;
+    if (tile0 >= SS) return;  // the whole block: past the stride's end
+    const int r = threadIdx.x / kSortRunThreads, t = threadIdx.x % kSortRunThreads;
+    uint64_t v[kSortE];
+    sort_tile_keys<kRun, kSortE>(key_grid(S), gl(io.in), SS, tile0 + r * kRun, ldc(ldc(S.mut)->u) * SS, v,
+                                 lds + r * kRun, t);
+    // coalesced store through LDS (thread i holds positions kSortE * i ..)
     __syncthreads();
-    const int nvalid = min(kTile, SS - tile0);
+#pragma unroll
+    for (int e = 0; e < kSortE; e++) lds[threadIdx.x * kSortE + e] = v[e];
+    __syncthreads();
+    const int nvalid = min(kSortBlockKeys, SS - tile0);
     const gptr<uint64_t> bkey0 = gl(ldc(S.bkey0));
     for (int i = threadIdx.x; i < nvalid; i += kTileThreads) bkey0[tile0 + i] = lds[i];
 }
@@ -2480,7 +2491,7 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
 #define PH_BEGIN(ph) if ((e = mark(ev, phase_mask, 2 * (ph), st)) != hipSuccess) return e
 #define PH_END(ph) if ((e = mark(ev, phase_mask, 2 * (ph) + 1, st)) != hipSuccess) return e
     PH_BEGIN(kPhStrideSort);
-    RTDBS_LAUNCH(k_sort_expire, sh.stiles + sh.wtiles, kTileThreads, a);
+    RTDBS_LAUNCH(k_sort_expire, sh.stiles * kSortParts + sh.wtiles, kTileThreads, a);
     PH_END(kPhStrideSort);
     PH_BEGIN(kPhMerge);
     if (sh.rank_split) {
