@@ -262,8 +262,8 @@ __device__ __forceinline__ uint64_t lane_minmax(uint64_t v, int m, bool keep_min
 // Et..Et+E-1 in v. Bitonic network: exchanges between a thread's own elements
 // stay in registers, exchanges within a wave use lane exchanges, and only the
 // stages whose partners sit in another wave (j >= 64E) go through LDS (`lds`,
-// N keys) with block barriers: none for N = 64E (one wave), one stage for a
-// 512-key run over two waves (E = 4), three for N = 2048 over four waves
+// N keys) with block barriers: none for N = 64E (one wave), three stages for
+// a 512-key run over four waves (E = 2) and for N = 2048 over four waves
 // (E = 8). Every thread of the block takes part in the same stages.
 template <int N, int E>
 __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[E], uint64_t *lds, int tid) {
@@ -385,13 +385,13 @@ __device__ __forceinline__ void expire_count(const BatchArgs &a, int g, int *lds
 // Launch 1 of a slide: the first kSortParts blocks of each stride tile sort
 // the stride by key (the reference's compute_cell_id + sortByCellIdAndOrder,
 // aabb.cu:589-611, thrust_helper.cu:21-29) in runs of kRun keys into bkey0,
-// two waves per run (kSortE keys per thread: half the network's instructions
-// per wave of a one-wave run, for one LDS stage); the remaining blocks count
-// the survivors of each window tile.
+// four waves per run (kSortE keys per thread: a quarter of the network's
+// instructions per wave of a one-wave run, for three LDS stages); the
+// remaining blocks count the survivors of each window tile.
 constexpr int kRun = 512;                                // keys per sorted run of the stride
-constexpr int kSortE = 4;                                // keys per thread
-constexpr int kSortRunThreads = kRun / kSortE;           // 128: two waves per run
-constexpr int kSortBlockKeys = kTileThreads * kSortE;    // 1024: two runs per block
+constexpr int kSortE = 2;                                // keys per thread
+constexpr int kSortRunThreads = kRun / kSortE;           // 256: four waves per run
+constexpr int kSortBlockKeys = kTileThreads * kSortE;    // 512: one run per block
 constexpr int kSortParts = kTile / kSortBlockKeys;       // blocks per stride tile
 
 __global__ __launch_bounds__(kTileThreads) void k_sort_expire(BatchArgs a) {
