@@ -578,7 +578,10 @@ int refresh_batch(rtdbs_batch *b) {
     b->shape.wtiles = pre[1 * (n + 1) + n];
     b->shape.chunks = pre[2 * (n + 1) + n];
     // stride rank: 64-key blocks when 256-key blocks would not cover the device
-    b->shape.rank_split = (long long)b->shape.stiles * (kTile / kTileThreads) < device_cus() ? 1 : 0;
+    // (RTDBS_RANK_SPLIT: 0 forces 256-key blocks, 1 the 64-key form)
+    const int rsplit = env_int("RTDBS_RANK_SPLIT", -1, 0, 1);
+    b->shape.rank_split =
+        rsplit >= 0 ? rsplit : ((long long)b->shape.stiles * (kTile / kTileThreads) < device_cus() ? 1 : 0);
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));

@@ -209,3 +209,21 @@ def test_neighbour_kernel_forms_agree(engine, oracle, monkeypatch, mode, dim, W,
     ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
     got = engine_slides(engine, data, dim, W, S, eps, mp, slides, cell_div=k)
     assert_parity(got, ref_l, ref_c, "mode %s dim%d W%d k%d" % (mode, dim, W, k))
+
+
+@pytest.mark.parametrize("split", ["0", "1"])
+@pytest.mark.parametrize("dim,W,S,eps,mp,seed", [
+    (2, 20000, 5000, 0.04, 6, 61),   # 3 stride tiles, 10 runs, a partial last run and tile
+    (3, 12288, 2048, 0.12, 8, 62),   # exactly one stride tile, 4 runs
+    (3, 5600, 700, 0.2, 8, 63),      # two runs, the second partial
+])
+def test_stride_rank_forms_agree(engine, oracle, monkeypatch, split, dim, W, S, eps, mp, seed):
+    """The stride rank's 256-key blocks (0) and 64-key split blocks (1) -
+    launches pick one by size, each is forced here - reproduce the oracle,
+    with strides of several sort runs and tiles, partial runs included."""
+    monkeypatch.setenv("RTDBS_RANK_SPLIT", split)
+    slides = 3
+    data = _mixed(W + slides * S, dim, seed)
+    ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
+    got = engine_slides(engine, data, dim, W, S, eps, mp, slides)
+    assert_parity(got, ref_l, ref_c, "rank split %s dim%d W%d S%d" % (split, dim, W, S))
