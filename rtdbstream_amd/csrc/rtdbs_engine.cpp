@@ -885,7 +885,10 @@ extern "C" int rtdbs_batch_set_timing(rtdbs_batch *b, int enable) {
     b->ring_used = 0;  // recorded slides of the previous setting are dropped
     if (enable && b->ev_ring.empty()) {
         b->ev_ring.assign((size_t)kTimingRing * 2 * kNumPhases, nullptr);
-        for (hipEvent_t &e : b->ev_ring) HIPCHK(hipEventCreate(&e));
+        // timing-only events: no system-scope fence (no cache writeback and
+        // invalidate around the timed kernel); they are read after the stream
+        // is synchronised
+        for (hipEvent_t &e : b->ev_ring) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     }
     // 1: every phase; 2: the range-count kernel only (two events per slide)
     b->timing = enable != 0;
