@@ -115,7 +115,8 @@ struct LaunchShape {
     int32_t persist_blocks;  // 256-thread blocks for the persistent first-core kernel
     int32_t rows_blocks;     // 256-thread blocks for the row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
-    int32_t lpq;             // lanes per query of the point-parallel kernels (1, 2 or 4)
+    int32_t lpq;             // lanes per query of the point-parallel count (1, 2 or 4)
+    int32_t lpq_border;      // lanes per query of the point-parallel output kernel
     int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels
     int32_t cellpar;         // 1: cell-parallel count/border, 0: point-parallel
     int32_t union_cells;     // point-parallel path: 1 cell-level merge kernels, 0 the cell-parallel merge

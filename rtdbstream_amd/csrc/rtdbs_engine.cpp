@@ -609,7 +609,13 @@ int refresh_batch(rtdbs_batch *b) {
     // unless a single stream asks for a tuned setting
     const long long lanes_wanted = (long long)device_cus() * 32 * 64 * 4;
     b->shape.lpq = cells_total * 4 <= lanes_wanted ? 4 : (cells_total * 2 <= lanes_wanted ? 2 : 1);
-    if (n == 1 && b->streams[0]->p.lanes_per_query > 0) b->shape.lpq = b->streams[0]->p.lanes_per_query;
+    // the output kernel walks every stencil row of a non-core point (no early
+    // exit), so its per-lane chains are longer: 4 lanes per query, also where
+    // the count takes 1 (64 streams: 238 vs 284 us; RTDBS_BORDER_LPQ=1|2|4
+    // overrides)
+    b->shape.lpq_border = env_int("RTDBS_BORDER_LPQ", 4, 1, 4);
+    if (n == 1 && b->streams[0]->p.lanes_per_query > 0)
+        b->shape.lpq = b->shape.lpq_border = b->streams[0]->p.lanes_per_query;
     (void)cells_total;
     BatchArgs &a = b->args;
     std::memset(&a, 0, sizeof(a));

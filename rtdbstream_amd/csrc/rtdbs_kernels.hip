@@ -2551,11 +2551,11 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.cellpar) RTDBS_LAUNCH(k_border<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.union_cells) {
-        if (sh.lpq >= 4) RTDBS_LAUNCH((k_border_points<4, false>), sh.chunks * 4, 64, a);
-        else if (sh.lpq == 2) RTDBS_LAUNCH((k_border_points<2, false>), sh.chunks * 2, 64, a);
+        if (sh.lpq_border >= 4) RTDBS_LAUNCH((k_border_points<4, false>), sh.chunks * 4, 64, a);
+        else if (sh.lpq_border == 2) RTDBS_LAUNCH((k_border_points<2, false>), sh.chunks * 2, 64, a);
         else RTDBS_LAUNCH((k_border_points<1, false>), sh.chunks, 64, a);
-    } else if (sh.lpq >= 4) RTDBS_LAUNCH((k_border_points<4, true>), sh.chunks * 4, 64, a);
-    else if (sh.lpq == 2) RTDBS_LAUNCH((k_border_points<2, true>), sh.chunks * 2, 64, a);
+    } else if (sh.lpq_border >= 4) RTDBS_LAUNCH((k_border_points<4, true>), sh.chunks * 4, 64, a);
+    else if (sh.lpq_border == 2) RTDBS_LAUNCH((k_border_points<2, true>), sh.chunks * 2, 64, a);
     else RTDBS_LAUNCH((k_border_points<1, true>), sh.chunks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
