@@ -227,3 +227,17 @@ def test_stride_rank_forms_agree(engine, oracle, monkeypatch, split, dim, W, S, 
     ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
     got = engine_slides(engine, data, dim, W, S, eps, mp, slides)
     assert_parity(got, ref_l, ref_c, "rank split %s dim%d W%d S%d" % (split, dim, W, S))
+
+
+@pytest.mark.parametrize("blpq", ["1", "2", "4"])
+def test_output_kernel_lanes_agree(engine, oracle, blpq, monkeypatch):
+    """The point-parallel output kernel on 1, 2 or 4 lanes per query, with the
+    count on its own setting (large batches pair a 1-lane count with a 4-lane
+    output kernel), reproduces the oracle."""
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", "2")
+    monkeypatch.setenv("RTDBS_BORDER_LPQ", blpq)
+    W, S, slides = 3000, 300, 4
+    data = _mixed(W + slides * S, 3, 44)
+    ref_l, ref_c = oracle.stream(data, W, S, 0.3, 6, slides, method="grid")
+    got = engine_slides(engine, data, 3, W, S, 0.3, 6, slides)
+    assert_parity(got, ref_l, ref_c, "output lanes %s" % blpq)
