@@ -1,9 +1,10 @@
 // HIP kernels of the MI355X streaming DBSCAN slide step (gfx950, wave64).
 //
-// One slide of every stream in a batch is the sequence
-//   stride_tiles -> merge_pass* -> expire_count -> expire_compact ->
-//   merge_final -> cells -> rows -> count -> union_sparse -> union_dense ->
-//   flatten -> border
+// One slide of every stream in a batch is eight launches:
+//   sort_expire (stride sort in runs + survivors per window tile) ->
+//   rank (stride rank merge + survivor compaction) -> merge_final -> cells ->
+//   rows -> count -> union -> border (outputs)
+// (large batches: count_points -> fcore -> rep -> union -> border_points).
 // Each kernel covers all streams of the batch in one launch: a block finds its
 // stream from prefix tables of per-stream work units (tiles of positions,
 // 64-position chunks) or, for the cell kernels whose sizes are only known on
@@ -1692,7 +1693,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
-    // the header comes screened (cell_heads); the rows load with the first
+    // the header comes screened (cell_head); the rows load with the first
     // queries, then each candidate chunk is a round of its own
     const int A = h.c;
     const bool safeA = h.cf & kCellSafe, denseA = h.cf & kCellDense;
