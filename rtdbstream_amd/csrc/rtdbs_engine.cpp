@@ -601,8 +601,12 @@ int refresh_batch(rtdbs_batch *b) {
     // the cell-parallel merge also wins in large batches (RTDBS_UNION_CELLS=1
     // selects the older cell-level merge kernels there)
     b->shape.union_cells = env_int("RTDBS_UNION_CELLS", 0, 0, 1);
+    // cell-parallel grid: 16 blocks per CU while the launch is small; 32 for
+    // large batches, where it carries the merge alone (64 streams: 455 ->
+    // ~400 us). RTDBS_CELLPAR_PER_CU overrides.
+    const int cp_def = cells_total <= kCellParMaxPoints ? 16 : 32;
     b->shape.cellpar_blocks = (int)std::max(
-        1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", 16, 1, 1024), (cells_total + 3) / 4));
+        1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", cp_def, 1, 1024), (cells_total + 3) / 4));
     b->shape.cell_blocks = (int)std::max(
         1ll, std::min((long long)device_cus() * waves_per_cu(), (cells_total + kUGCells - 1) / kUGCells));
     // lanes per query: enough threads to cover the device several times over,
