@@ -1,27 +1,38 @@
 """Benchmark of the MI355X streaming DBSCAN slide step.
 
-Default workload (the configuration the headline metric is quoted on): the
-3-D trajectory-like stream "walks3d" - W 100,000, S 10,000, Eps 0.5, MinPts 10,
-one stream, plain per-slide GPU path. A step is one slide: insert the stride,
-expire the oldest stride, recluster the window, labels available on the device.
-All inputs are resident in HBM before the timed region.
+Headline (`value`): the 64-stream batch - 64 independent 2-D/3-D streams,
+W log-uniform in [5,000, 200,000], S = W/10, Eps in [0.3, 1.2], MinPts in
+{5, 10, 20} (rtdbstream_amd/workloads.py multi_stream_specs) - all sliding
+together, one HIP-graph replay per slide. A step is one slide of every
+stream: insert each stride, expire the oldest, recluster every window,
+labels and ids written. Inputs (all strides) are resident in HBM when the
+timed region starts and outputs stay in HBM; `value` is stride points
+ingested per second over the whole batch. `vs_plain` compares it with the same
+streams slid one at a time through the plain path on the same GPU.
 
-Also reported on the same JSON line:
-  roofline      - the range-count kernel: algorithmic bytes per launch
-                  (rtdbstream_amd/roofline.py) / its average duration from HIP
-                  events recorded by its own dispatch; `traffic` is the HBM
-                  bytes per launch from the rocprofv3 counter passes summarised
-                  in profiles/k_count_<workload>_traffic.md
-  cpu_baseline  - the CPU port (oracle, OpenMP) on a bounded sample of the same
-                  slides, whose labels are also compared with the GPU's
-  batched       - the 64-stream batch: one launch per phase, HIP graph, vs the
-                  same streams slid one at a time (plain)
-  served        - the request server under Poisson arrivals: throughput and
-                  p50/p99 latency
+Also on the same JSON line:
+  transfer_inclusive - the same two legs timed the reference's way
+                  (dbscan.cpp:2289 stride H2D through :2344-2345 label/cid
+                  D2H, into pinned host buffers, :54-55): strides from pinned
+                  host memory, labels and ids back in pinned host memory
+  plain         - single-stream legs: walks3d (W 100,000) and skewed3d
+                  (W 1,000,000), device-resident and transfer-inclusive
+  roofline      - the range-count kernel of the 64-stream launch: algorithmic
+                  bytes per launch (rtdbstream_amd/roofline.py) / its average
+                  duration from HIP events recorded by its own dispatch on the
+                  batch's stream; `traffic` is the HBM bytes per launch from
+                  the rocprofv3 counter passes summarised under profiles/
+  cpu_baseline  - the CPU port (oracle grid restatement, OpenMP on all host
+                  threads) on a bounded sample of the 64-stream workload,
+                  labels compared; plus the single-thread brute force of the
+                  reference's CPU path on blobs2d (W 10,000)
+  served        - the request server (256 streams, Poisson arrivals at a
+                  swept rate): throughput and p50/p99 latency, host buffers
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -38,6 +49,16 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def window_at(data, W, S, t):
     win = np.array(data[:W], copy=True)
     units = W // S
@@ -45,6 +66,10 @@ def window_at(data, W, S, t):
         u = k % units
         win[u * S:(u + 1) * S] = data[W + k * S: W + (k + 1) * S]
     return win
+
+
+def pct(v, q):
+    return round(float(np.percentile(v, q)), 4)
 
 
 def tune(R, dim, W, S, eps, mp, data, enabled):
@@ -55,26 +80,256 @@ def tune(R, dim, W, S, eps, mp, data, enabled):
     return R.autotune(dim, W, S, eps, mp, data[:W + 9 * S], slides=8)
 
 
-def tuned_stream(R, dim, W, S, eps, mp, t):
-    return R.Stream(dim, W, S, eps, mp, cell_div=t["cell_div"], lanes_per_query=t["lanes_per_query"],
-                    neighbor_mode=t["neighbor_mode"])
+def tuned_stream(R, sp, t, data):
+    s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], cell_div=t["cell_div"],
+                 lanes_per_query=t["lanes_per_query"], neighbor_mode=t["neighbor_mode"])
+    s.init(data[:sp["window"]])
+    return s
 
 
-def plain_leg(R, WL, name, steps, warmup, autotune=True):
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel on a workload, from the two rocprofv3
+    counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by what it reads for a
+    copy kernel of known size) summarised under profiles/<name>_traffic.md;
+    None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.md" % name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def range_count_bytes(specs, datas, infos, ts):
+    """Algorithmic bytes of one range-count launch over the given streams,
+    averaged over the windows of slides ts."""
+    from rtdbstream_amd import roofline as RF
+
+    total = 0.0
+    for sp, d, info in zip(specs, datas, infos):
+        W, S = sp["window"], sp["stride"]
+        per = []
+        for t in ts:
+            win = window_at(d, W, S, t)
+            q = ~RF.dense_mask(win, sp["eps"], sp["min_pts"], info)
+            per.append(RF.range_count_bytes(win, sp["eps"], q))
+        total += float(np.mean(per))
+    return total
+
+
+def roofline_obj(B, avg_ms, kernel, traffic_name):
+    achieved = B / (avg_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(traffic_name),
+            "kernel": kernel, "algorithmic_bytes_per_launch": int(B), "avg_launch_us": round(avg_ms * 1e3, 2)}
+
+
+class PinnedSlides:
+    """Pinned host buffers of the transfer-inclusive legs: per slide, every
+    stream's stride one after the other (the layout a batched receive would
+    hand over), and per stream its label then cid, streams in order - the
+    layout in which a batch's copies merge into one each way."""
+
+    def __init__(self, R, specs, datas, t0, nslides):
+        self.specs = specs
+        in_b = sum(sp["stride"] * sp["dim"] * 4 for sp in specs)
+        out_b = sum(2 * sp["window"] * 4 for sp in specs)
+        self.hin = R.HostBuffer(in_b * nslides)
+        self.hout = R.HostBuffer(out_b)
+        self.ins = []
+        o = 0
+        for t in range(t0, t0 + nslides):
+            row = []
+            for sp, d in zip(specs, datas):
+                W, S, dim = sp["window"], sp["stride"], sp["dim"]
+                a = self.hin.array(np.float32, S * dim, offset=o)
+                a[:] = d[W + t * S: W + (t + 1) * S].reshape(-1)
+                row.append(a)
+                o += S * dim * 4
+            self.ins.append(row)
+        self.labels, self.cids = [], []
+        o = 0
+        for sp in specs:
+            W = sp["window"]
+            self.labels.append(self.hout.array(np.int32, W, offset=o))
+            self.cids.append(self.hout.array(np.int32, W, offset=o + W * 4))
+            o += 2 * W * 4
+
+    def free(self):
+        self.hin.free()
+        self.hout.free()
+
+
+def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
+    """The 64-stream workload: plain (one stream at a time) and batched (one
+    graph replay per slide), each device-resident and transfer-inclusive."""
+    extra = 3
+    total = warmup + 2 * steps + extra
+    specs = WL.multi_stream_specs(n_streams=n_streams, slides=total + 1)
+    datas = [WL.stream_points(sp) for sp in specs]
+    bufs = []
+    for sp, d in zip(specs, datas):
+        db = R.DeviceBuffer(d.nbytes)
+        db.upload(d)
+        W = sp["window"]
+        bufs.append((db, R.DeviceBuffer(W * 4), R.DeviceBuffer(W * 4), R.DeviceBuffer(W * 4), R.DeviceBuffer(W * 4)))
+    t0 = time.perf_counter()
+    tuned = [tune(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], d, autotune)
+             for sp, d in zip(specs, datas)]
+    tune_s = time.perf_counter() - t0
+
+    def in_addr(i, t):
+        sp = specs[i]
+        return bufs[i][0].addr((sp["window"] + t * sp["stride"]) * sp["dim"] * 4)
+
+    pts_per_slide = sum(sp["stride"] for sp in specs)
+    out = {"streams": len(specs), "window_points": sum(sp["window"] for sp in specs),
+           "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
+           "stride_points_per_slide": pts_per_slide}
+
+    # --- device-resident, plain: slides 0 .. warmup+steps-1
+    streams = [tuned_stream(R, sp, tu, d) for sp, d, tu in zip(specs, datas, tuned)]
+    for t in range(warmup):
+        for i, s in enumerate(streams):
+            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
+            s.sync()
+    R.lib().rtdbs_device_synchronize()
+    lat_plain = []
+    a0 = time.perf_counter()
+    for t in range(warmup, warmup + steps):
+        for i, s in enumerate(streams):
+            a = time.perf_counter()
+            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
+            s.sync()
+            lat_plain.append((time.perf_counter() - a) * 1e3)
+    R.lib().rtdbs_device_synchronize()
+    t_plain = time.perf_counter() - a0
+    # --- transfer-inclusive, plain: the next `steps` slides from pinned host buffers
+    tx = PinnedSlides(R, specs, datas, warmup + steps, steps)
+    lat_plain_x = []
+    R.lib().rtdbs_device_synchronize()
+    a0 = time.perf_counter()
+    for k in range(steps):
+        for i, s in enumerate(streams):
+            a = time.perf_counter()
+            s.slide(tx.ins[k][i], tx.labels[i], tx.cids[i])
+            lat_plain_x.append((time.perf_counter() - a) * 1e3)
+    t_plain_x = time.perf_counter() - a0
+    plain_last = [(l.copy(), c.copy()) for l, c in zip(tx.labels, tx.cids)]
+    infos = [s.info() for s in streams]
+    for s in streams:
+        s.close()
+
+    # --- batched, same streams from the same first windows
+    streams = [tuned_stream(R, sp, tu, d) for sp, d, tu in zip(specs, datas, tuned)]
+    b = R.Batch(streams, use_graph=True)
+
+    def bslide(t):
+        b.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                      [x[4].addr() for x in bufs])
+        b.sync()
+
+    for t in range(warmup):
+        bslide(t)
+    R.lib().rtdbs_device_synchronize()
+    lat_b = []
+    a0 = time.perf_counter()
+    for t in range(warmup, warmup + steps):
+        a = time.perf_counter()
+        bslide(t)
+        lat_b.append((time.perf_counter() - a) * 1e3)
+    R.lib().rtdbs_device_synchronize()
+    t_batch = time.perf_counter() - a0
+    same_dev = True
+    for i, sp in enumerate(specs):
+        W = sp["window"]
+        l1, c1, l2, c2 = (np.empty(W, np.int32) for _ in range(4))
+        bufs[i][1].download(l1)
+        bufs[i][2].download(c1)
+        bufs[i][3].download(l2)
+        bufs[i][4].download(c2)
+        same_dev = same_dev and np.array_equal(l1, l2) and np.array_equal(c1, c2)
+    # --- transfer-inclusive, batched: the same pinned buffers (two copies per slide)
+    lat_bx = []
+    R.lib().rtdbs_device_synchronize()
+    a0 = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        b.slide(tx.ins[k], tx.labels, tx.cids)
+        lat_bx.append((time.perf_counter() - a) * 1e3)
+    t_batch_x = time.perf_counter() - a0
+    same_x = all(np.array_equal(l, pl) and np.array_equal(c, pc)
+                 for l, c, (pl, pc) in zip(tx.labels, tx.cids, plain_last))
+    tx.free()
+    # --- range-count kernel of the batched launch: HIP events on the batch's
+    # own stream, a few more slides without the graph
+    bt = R.Batch(streams, use_graph=False)
+    bt.set_timing(2)
+    t_extra = list(range(warmup + 2 * steps, warmup + 2 * steps + extra))
+    for t in t_extra:
+        bt.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                       [x[4].addr() for x in bufs])
+        bt.sync()
+    cnt_ms, cnt_n = bt.timing()["count"]
+    bt.set_timing(1)  # per-phase breakdown (events between phases: outside the timed region)
+    t_ph = warmup + 2 * steps + extra
+    bt.slide_async([in_addr(i, t_ph) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                   [x[4].addr() for x in bufs])
+    bt.sync()
+    phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in bt.timing().items() if v[1]}
+    b.close()
+    bt.close()
+    for s in streams:
+        s.close()
+    forms = {}
+    for tu in tuned:
+        f = "k%d/lpq%d/mode%d" % (tu["cell_div"], tu["lanes_per_query"], tu["neighbor_mode"])
+        forms[f] = forms.get(f, 0) + 1
+    pts = pts_per_slide * steps
+    nsl = len(specs) * steps
+    out.update({
+        "autotune": {"forms": forms, "from_cache": sum(bool(tu["from_cache"]) for tu in tuned),
+                     "tune_s": round(tune_s, 2)},
+        "t_batch": t_batch, "steps": steps,
+        "plain_points_per_s": round(pts / t_plain, 1), "plain_slides_per_s": round(nsl / t_plain, 1),
+        "batched_points_per_s": round(pts / t_batch, 1), "batched_slides_per_s": round(nsl / t_batch, 1),
+        "speedup": round(t_plain / t_batch, 3),
+        "plain_p50_ms": pct(lat_plain, 50), "plain_p99_ms": pct(lat_plain, 99),
+        "batched_step_p50_ms": pct(lat_b, 50), "batched_step_p99_ms": pct(lat_b, 99),
+        "labels_equal_plain": bool(same_dev),
+        "phase_ms": phase_ms,
+        "transfer_inclusive": {
+            "plain_points_per_s": round(pts / t_plain_x, 1), "plain_slides_per_s": round(nsl / t_plain_x, 1),
+            "batched_points_per_s": round(pts / t_batch_x, 1), "batched_slides_per_s": round(nsl / t_batch_x, 1),
+            "speedup": round(t_plain_x / t_batch_x, 3),
+            "plain_p50_ms": pct(lat_plain_x, 50), "plain_p99_ms": pct(lat_plain_x, 99),
+            "batched_step_p50_ms": pct(lat_bx, 50), "batched_step_p99_ms": pct(lat_bx, 99),
+            "labels_equal_plain": bool(same_x),
+            "host_buffers": "pinned (rtdbs_host_alloc); per slide one stride H2D and one label+cid D2H "
+                            "for the batch, three copies per plain slide"},
+        "_count": (cnt_ms / max(1, cnt_n), t_extra, specs, datas, infos),
+    })
+    return out
+
+
+def plain_leg(R, WL, name, steps, warmup, autotune=True, roof_sample=4):
+    """One stream of a named workload: device-resident slides (timed, with
+    the range-count kernel's own events), then the same number of slides
+    with host buffers in pinned memory."""
     cfg = WL.WORKLOADS[name]
-    W, S, dim, eps, mp = cfg["window"], cfg["stride"], cfg["dim"], cfg["eps"], cfg["min_pts"]
-    total_slides = warmup + steps
-    extra = 10
-    data = WL.generate(name, n=W + (total_slides + extra) * S)
+    sp = dict(cfg, index=0, gen=name)
+    W, S, dim = cfg["window"], cfg["stride"], cfg["dim"]
+    extra = 3
+    total = warmup + 2 * steps + extra
+    data = WL.generate(name, n=W + total * S)
     d_data = R.DeviceBuffer(data.nbytes)
     d_data.upload(data)
-    d_label = R.DeviceBuffer(W * 4)
-    d_cid = R.DeviceBuffer(W * 4)
+    d_label, d_cid = R.DeviceBuffer(W * 4), R.DeviceBuffer(W * 4)
     t0 = time.perf_counter()
-    tuned = tune(R, dim, W, S, eps, mp, data, autotune)
+    tu = tune(R, dim, W, S, cfg["eps"], cfg["min_pts"], data, autotune)
     tune_s = time.perf_counter() - t0
-    s = tuned_stream(R, dim, W, S, eps, mp, tuned)
-    s.init(data[:W])
+    s = tuned_stream(R, sp, tu, data)
 
     def slide(t):
         s.slide_async(d_data.addr((W + t * S) * dim * 4), d_label.addr(), d_cid.addr())
@@ -85,236 +340,111 @@ def plain_leg(R, WL, name, steps, warmup, autotune=True):
     s.set_timing(2)  # events around the range-count kernel only
     lat = []
     R.lib().rtdbs_device_synchronize()
-    t0 = time.perf_counter()
-    for t in range(warmup, total_slides):
+    a0 = time.perf_counter()
+    for t in range(warmup, warmup + steps):
         a = time.perf_counter()
         slide(t)
         lat.append((time.perf_counter() - a) * 1e3)
     R.lib().rtdbs_device_synchronize()
-    elapsed = time.perf_counter() - t0
+    el = time.perf_counter() - a0
     timing = s.timing()
+    s.set_timing(0)
     info = s.info()
-    # per-phase breakdown from a separate pass (events between every phase
-    # add launch gaps, so they stay out of the timed region)
+    # transfer-inclusive: pinned stride in, pinned label + cid out
+    tx = PinnedSlides(R, [sp], [data], warmup + steps, steps)
+    latx = []
+    a0 = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        s.slide(tx.ins[k][0], tx.labels[0], tx.cids[0])
+        latx.append((time.perf_counter() - a) * 1e3)
+    elx = time.perf_counter() - a0
+    tx.free()
     s.set_timing(1)
-    for t in range(total_slides, total_slides + min(steps, extra)):
+    for t in range(warmup + 2 * steps, warmup + 2 * steps + extra):
         slide(t)
-    phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in s.timing().items()}
+    phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in s.timing().items() if v[1]}
     s.close()
-    return dict(data=data, cfg=cfg, cfg_name=name, elapsed=elapsed, lat=lat, timing=timing, info=info, W=W, S=S, dim=dim,
-                phase_ms=phase_ms, tuned={k: tuned[k] for k in ("cell_div", "lanes_per_query", "neighbor_mode",
-                                                                 "from_cache", "trials")},
-                tune_s=round(tune_s, 2),
-                eps=eps, min_pts=mp, warmup=warmup, steps=steps)
+    ts = list(range(warmup, warmup + steps))
+    ts = ts[:: max(1, len(ts) // roof_sample)][:roof_sample]
+    ms, n = timing["count"]
+    B = range_count_bytes([sp], [data], [info], ts)
+    return {
+        "workload": name, "dim": dim, "window": W, "stride": S, "eps": cfg["eps"], "min_pts": cfg["min_pts"],
+        "steps": steps,
+        "points_per_s": round(steps * S / el, 1), "slides_per_s": round(steps / el, 2),
+        "ms_per_slide": round(el / steps * 1e3, 4), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "transfer_inclusive": {"points_per_s": round(steps * S / elx, 1), "ms_per_slide": round(elx / steps * 1e3, 4),
+                               "p50_ms": pct(latx, 50), "p99_ms": pct(latx, 99)},
+        "phase_ms": phase_ms,
+        "autotune": {k: tu[k] for k in ("cell_div", "lanes_per_query", "neighbor_mode", "from_cache", "trials")},
+        "tune_s": round(tune_s, 2),
+        "cells_last": info["cells"],
+        "roofline": roofline_obj(B, ms / max(1, n), "k_count", "k_count_%s" % name),
+    }
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of the range-count kernel on this workload, from
-    the two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by
-    what it reads for a copy kernel of known size) summarised under profiles/.
-    None when no such summary exists for the workload."""
-    path = os.path.join(ROOT, "profiles", "k_count_%s_traffic.md" % workload)
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
-
-
-def roofline(WL, res, sample=8):
-    from rtdbstream_amd import roofline as RF
-
-    data, W, S, eps, mp = res["data"], res["W"], res["S"], res["eps"], res["min_pts"]
-    info = res["info"]
-    t_first = res["warmup"]
-    ts = list(range(t_first, t_first + res["steps"]))
-    if len(ts) > sample:
-        ts = ts[:: max(1, len(ts) // sample)][:sample]
-    bytes_per = []
-    for t in ts:
-        win = window_at(data, W, S, t)
-        q = ~RF.dense_mask(win, eps, mp, info)
-        bytes_per.append(RF.range_count_bytes(win, eps, q))
-    B = float(np.mean(bytes_per))
-    ms, n = res["timing"]["count"]
-    avg_s = ms / max(1, n) / 1e3
-    achieved = B / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(res["cfg_name"]),
-            "kernel": "k_count", "algorithmic_bytes_per_launch": int(B),
-            "avg_launch_us": round(avg_s * 1e6, 2)}
-
-
-def batched_roofline(WL, bres):
-    """Range-count roofline of the 64-stream launch: algorithmic bytes summed
-    over the streams' windows / the launch's HIP-event duration."""
-    from rtdbstream_amd import roofline as RF
-
-    avg_ms, ts, specs, datas, infos = bres.pop("_count_kernel")
-    total = 0.0
-    for sp, d, info in zip(specs, datas, infos):
-        W, S = sp["window"], sp["stride"]
-        per = []
-        for t in ts:
-            win = window_at(d, W, S, t)
-            q = ~RF.dense_mask(win, sp["eps"], sp["min_pts"], info)
-            per.append(RF.range_count_bytes(win, sp["eps"], q))
-        total += float(np.mean(per))
-    achieved = total / (avg_ms / 1e3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_count (64-stream launch)",
-            "algorithmic_bytes_per_launch": int(total), "avg_launch_us": round(avg_ms * 1e3, 2)}
-
-
-def cpu_baseline(R, WL, res, seconds, max_slides=400):
-    """CPU port (oracle grid, OpenMP) on a bounded sample of the same workload
-    (about `seconds` of CPU work); the GPU labels of the same slides are
-    computed alongside and compared bit for bit."""
+def cpu_baseline(R, WL, seconds, bf_seconds):
+    """CPU port (oracle grid restatement, OpenMP) on a bounded sample of the
+    64-stream workload: slides of the streams in turn, each stream's GPU
+    labels compared bit for bit. Then the single-thread brute force of the
+    reference's CPU path (util.cpp:299-354) on blobs2d."""
     from oracle import oracle as O
 
-    W, S, dim, eps, mp = res["W"], res["S"], res["dim"], res["eps"], res["min_pts"]
-    name = res["cfg_name"]
-    data = WL.generate(name, n=W + max_slides * S)
     threads = O.max_threads()
-    s = R.Stream(dim, W, S, eps, mp)
-    s.init(data[:W])
-    win = np.array(data[:W], copy=True)
-    units = W // S
-    n_done, cpu_time, exact = 0, 0.0, True
-    t = 0
-    while (cpu_time < seconds or n_done < 2) and t < max_slides:
-        stride = data[W + t * S: W + (t + 1) * S]
-        label, cid = s.slide(stride)
-        u = t % units
-        win[u * S:(u + 1) * S] = stride
-        a = time.perf_counter()
-        rl, rc = O.grid(win, eps, mp, threads=threads)
-        cpu_time += time.perf_counter() - a
-        exact = exact and np.array_equal(label, rl) and np.array_equal(cid, rc)
-        n_done += 1
-        t += 1
-    s.close()
-    return {"value": round(n_done * S / cpu_time, 1), "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": "%d slides of %s (W=%d, S=%d): oracle grid restatement, OpenMP on %d threads, %.1f s"
-                      % (n_done, name, W, S, threads, cpu_time),
-            "slides_per_s": round(n_done / cpu_time, 3)}, exact, n_done
-
-
-def batched_leg(R, WL, n_streams, slides, warmup, autotune=True):
-    specs = WL.multi_stream_specs(n_streams=n_streams, slides=slides + warmup + 2)
-    streams, bufs, datas = [], [], []
-    for sp in specs:
-        d = WL.stream_points(sp)
-        datas.append(d)
-        db = R.DeviceBuffer(d.nbytes)
-        db.upload(d)
-        bufs.append((db, R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4),
-                     R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4)))
-    # every stream's grid side and kernel form from the autotuner (both legs)
-    t0 = time.perf_counter()
-    tuned = [tune(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], d, autotune)
-             for sp, d in zip(specs, datas)]
-    tune_s = time.perf_counter() - t0
-    # plain: one stream at a time
-    for sp, d, tu in zip(specs, datas, tuned):
-        s = tuned_stream(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], tu)
-        s.init(d[:sp["window"]])
-        streams.append(s)
-
-    def in_addr(i, t):
+    per = 10  # slides per stream at most
+    specs = WL.multi_stream_specs(slides=per)
+    order = np.random.default_rng(0).permutation(len(specs))
+    n_done, n_streams, pts, cpu_time, exact = 0, 0, 0, 0.0, True
+    for i in order:
+        if cpu_time >= seconds:
+            break
         sp = specs[i]
-        return bufs[i][0].addr((sp["window"] + t * sp["stride"]) * sp["dim"] * 4)
-
-    for t in range(warmup):
-        for i, s in enumerate(streams):
-            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
-            s.sync()
-    R.lib().rtdbs_device_synchronize()
-    t0 = time.perf_counter()
-    lat_plain = []
-    for t in range(warmup, warmup + slides):
-        for i, s in enumerate(streams):
+        d = WL.stream_points(sp)
+        W, S = sp["window"], sp["stride"]
+        s = R.Stream(sp["dim"], W, S, sp["eps"], sp["min_pts"])
+        s.init(d[:W])
+        n_streams += 1
+        for t in range(per):
+            if cpu_time >= seconds and t > 0:
+                break
+            label, cid = s.slide(d[W + t * S: W + (t + 1) * S])
+            win = window_at(d, W, S, t)
             a = time.perf_counter()
-            s.slide_async(in_addr(i, t), bufs[i][1].addr(), bufs[i][2].addr())
-            s.sync()
-            lat_plain.append((time.perf_counter() - a) * 1e3)
-    R.lib().rtdbs_device_synchronize()
-    t_plain = time.perf_counter() - t0
-    for s in streams:
+            rl, rc = O.grid(win, sp["eps"], sp["min_pts"], threads=threads)
+            cpu_time += time.perf_counter() - a
+            exact = exact and np.array_equal(label, rl) and np.array_equal(cid, rc)
+            n_done += 1
+            pts += S
         s.close()
-    # batched: same streams from the same first windows, one graph replay per slide
-    streams = []
-    for sp, d, tu in zip(specs, datas, tuned):
-        s = tuned_stream(R, sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], tu)
-        s.init(d[:sp["window"]])
-        streams.append(s)
-    b = R.Batch(streams, use_graph=True)
-    bt = R.Batch(streams, use_graph=False)  # same streams, events around the range count
-
-    def bslide(t):
-        b.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
-                      [x[4].addr() for x in bufs])
-        b.sync()
-
-    for t in range(warmup):
-        bslide(t)
-    R.lib().rtdbs_device_synchronize()
-    t0 = time.perf_counter()
-    lat_b = []
-    for t in range(warmup, warmup + slides):
+    out = {"value": round(pts / cpu_time, 1), "unit": "points/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "host_threads_visible": os.cpu_count(),
+           "sample": "%d slides of the 64-stream workload (%d streams in a seeded order, up to %d slides each): "
+                     "oracle grid restatement, OpenMP on %d threads, %.1f s" % (n_done, n_streams, per, threads,
+                                                                               cpu_time),
+           "slides_per_s": round(n_done / cpu_time, 3), "labels_equal_gpu": bool(exact)}
+    # single-thread brute force (O(W^2)), blobs2d W 10,000
+    cfg = WL.WORKLOADS["blobs2d"]
+    W, S = cfg["window"], cfg["stride"]
+    data = WL.generate("blobs2d")
+    s = R.Stream(2, W, S, cfg["eps"], cfg["min_pts"])
+    s.init(data[:W])
+    nb, bt, ex2 = 0, 0.0, True
+    while (bt < bf_seconds or nb < 1) and nb < cfg["slides"]:
+        label, cid = s.slide(data[W + nb * S: W + (nb + 1) * S])
+        win = window_at(data, W, S, nb)
         a = time.perf_counter()
-        bslide(t)
-        lat_b.append((time.perf_counter() - a) * 1e3)
-    R.lib().rtdbs_device_synchronize()
-    t_batch = time.perf_counter() - t0
-    # exact: batched labels of the last slide equal the plain ones, per stream
-    # (checked before the timing pass below moves the windows on)
-    same = True
-    for i, sp in enumerate(specs):
-        W = sp["window"]
-        l1, c1, l2, c2 = (np.empty(W, np.int32) for _ in range(4))
-        bufs[i][1].download(l1)
-        bufs[i][2].download(c1)
-        bufs[i][3].download(l2)
-        bufs[i][4].download(c2)
-        same = same and np.array_equal(l1, l2) and np.array_equal(c1, c2)
-    # range-count kernel of the batched launch: HIP events on the batch's own
-    # stream, two more slides without the graph
-    bt.set_timing(2)
-    t_extra = []
-    for t in range(warmup + slides, warmup + slides + 2):
-        bt.slide_async([in_addr(i, t) for i in range(len(streams))], [x[3].addr() for x in bufs],
-                       [x[4].addr() for x in bufs])
-        bt.sync()
-        t_extra.append(t)
-    cnt_ms, cnt_n = bt.timing()["count"]
-    infos = [st.info() for st in streams]
-    b.close()
-    bt.close()
-    for s in streams:
-        s.close()
-    pts = sum(sp["stride"] for sp in specs) * slides
-    nsl = len(specs) * slides
-    forms = {}
-    for tu in tuned:
-        f = "k%d/lpq%d/mode%d" % (tu["cell_div"], tu["lanes_per_query"], tu["neighbor_mode"])
-        forms[f] = forms.get(f, 0) + 1
-    return {
-        "streams": len(specs), "slides_per_stream": slides,
-        "autotune": {"forms": forms, "from_cache": sum(bool(tu["from_cache"]) for tu in tuned),
-                     "tune_s": round(tune_s, 2)},
-        "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
-        "plain_points_per_s": round(pts / t_plain, 1), "plain_slides_per_s": round(nsl / t_plain, 1),
-        "batched_points_per_s": round(pts / t_batch, 1), "batched_slides_per_s": round(nsl / t_batch, 1),
-        "speedup": round(t_plain / t_batch, 3),
-        "plain_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
-        "plain_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
-        "batched_step_p50_ms": round(float(np.percentile(lat_b, 50)), 3),
-        "batched_step_p99_ms": round(float(np.percentile(lat_b, 99)), 3),
-        "labels_equal_plain": bool(same),
-        "_count_kernel": (cnt_ms / max(1, cnt_n), t_extra, specs, datas, infos),
-    }
+        rl, rc = O.bruteforce(win, cfg["eps"], cfg["min_pts"])
+        bt += time.perf_counter() - a
+        ex2 = ex2 and np.array_equal(label, rl) and np.array_equal(cid, rc)
+        nb += 1
+    s.close()
+    out["bruteforce_blobs2d"] = {"value": round(nb * S / bt, 1), "unit": "points/s", "cores": 1,
+                                 "ms_per_slide": round(bt / nb * 1e3, 2), "slides": nb,
+                                 "labels_equal_gpu": bool(ex2),
+                                 "sample": "%d slides of blobs2d (W %d, S %d): literal restatement of the "
+                                           "reference's CPU DBSCAN, one thread, %.1f s" % (nb, W, S, bt)}
+    return out
 
 
 def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, seed=7):
@@ -346,8 +476,7 @@ def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, s
            "hbm_budget_gb": budget_gb, "max_batch": max_batch, "lanes": lanes,
            "plain_serial_slides_per_s": round(capacity, 1),
            "plain_serial_points_per_s": round(sum(sp["stride"] for sp in specs) * slides / plain_total_s, 1),
-           "plain_p50_ms": round(float(np.percentile(plain_ms, 50)), 3),
-           "plain_p99_ms": round(float(np.percentile(plain_ms, 99)), 3), "sweep": []}
+           "plain_p50_ms": pct(plain_ms, 50), "plain_p99_ms": pct(plain_ms, 99), "sweep": []}
     rng = np.random.default_rng(seed)
     exact = True
     for f in factors:
@@ -396,20 +525,21 @@ def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, s
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="walks3d")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--steps", type=int, default=10, help="timed slides of the 64-stream batch")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch-streams", type=int, default=64)
-    ap.add_argument("--batch-slides", type=int, default=10)
-    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=30, help="timed slides of each single-stream leg")
+    ap.add_argument("--plain", default="walks3d,skewed3d", help="single-stream legs ('' for none)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--bf-seconds", type=float, default=5.0)
+    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-streams", type=int, default=256)
     ap.add_argument("--serve-slides", type=int, default=3)
     ap.add_argument("--serve-rates", default="0.5,1,2,4")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this size (profiler byte-counter calibration)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="run only this leg (batched, plain, cpu, served) - for profiling")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default grid side and kernel form instead of the autotuner's choice")
     args = ap.parse_args()
@@ -431,45 +561,50 @@ def main():
         R._check(R.lib().rtdbs_device_copy(R.ctypes.c_void_p(dst.addr()), R.ctypes.c_void_p(src.addr()), nb))
         log("calibration copy: %d MiB read + %d MiB written by k_copy16" % (args.calib_copy_mb, args.calib_copy_mb))
         del src, dst
-    log("plain leg: %s, %d warmup + %d timed slides" % (args.workload, args.warmup, args.steps))
-    res = plain_leg(R, WL, args.workload, args.steps, args.warmup, autotune=not args.no_autotune)
-    W, S = res["W"], res["S"]
-    ms_per = res["elapsed"] / args.steps * 1e3
-    out = {
-        "metric": METRIC,
-        "value": round(args.steps * S / res["elapsed"], 1),
-        "unit": "points/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_per, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic: seeded %s generator (rtdbstream_amd/workloads.py), inputs resident in HBM" % args.workload,
-        "config": {"workload": args.workload, "dim": res["dim"], "window": W, "stride": S, "eps": res["eps"],
-                   "min_pts": res["min_pts"], "streams": 1, "path": "plain"},
-        "slides_per_s": round(args.steps / res["elapsed"], 2),
-        "window_points_per_s": round(args.steps * W / res["elapsed"], 1),
-        "p50_ms": round(float(np.percentile(res["lat"], 50)), 4),
-        "p99_ms": round(float(np.percentile(res["lat"], 99)), 4),
-        "phase_ms": res["phase_ms"],
-        "grid": {"cell_div": res["info"]["cell_div"], "cells_last": res["info"]["cells"]},
-        "autotune": dict(res["tuned"], tune_s=res["tune_s"]),
-    }
-    log("plain: %.1f points/s, %.3f ms/slide" % (out["value"], ms_per))
-    out["roofline"] = roofline(WL, res)
-    log("roofline: %s" % out["roofline"])
-    if not args.no_cpu:
-        cb, exact, nchk = cpu_baseline(R, WL, res, args.cpu_seconds)
-        out["cpu_baseline"] = cb
-        out["exact_labels"] = {"slides_checked": nchk, "equal": bool(exact)}
-        log("cpu baseline: %s exact=%s" % (cb, exact))
-    if not args.no_batch:
-        log("batched leg: %d streams x %d slides" % (args.batch_streams, args.batch_slides))
-        out["batched"] = batched_leg(R, WL, args.batch_streams, args.batch_slides, 2,
-                                     autotune=not args.no_autotune)
-        out["roofline_batched"] = batched_roofline(WL, out["batched"])
-        log("batched: %s" % out["batched"])
-        log("roofline (batched launch): %s" % out["roofline_batched"])
-    if not args.no_serve:
+    only = args.only
+    autotune = not args.no_autotune
+    out = {"metric": METRIC}
+    if not only or only == "batched":
+        log("batched leg: %d streams, %d warmup + %d timed slides" % (args.batch_streams, args.warmup, args.steps))
+        bres = batched_leg(R, WL, args.batch_streams, args.steps, args.warmup, autotune)
+        avg_ms, ts, specs, datas, infos = bres.pop("_count")
+        t_batch = bres.pop("t_batch")
+        B = range_count_bytes(specs, datas, infos, ts)
+        out.update({
+            "value": bres["batched_points_per_s"], "unit": "points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(t_batch / args.steps * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32",
+            "data": "synthetic: 64 seeded streams (rtdbstream_amd/workloads.py multi_stream_specs), "
+                    "strides resident in HBM, labels and ids written to HBM",
+            "config": {"workload": "64 batched streams: 2-D blobs / 3-D walks, W %d-%d (%d window points), "
+                                   "S = W/10, Eps 0.3-1.2, MinPts 5/10/20, one graph replay per slide"
+                                   % (bres["window_range"][0], bres["window_range"][1], bres["window_points"]),
+                       "streams": bres["streams"], "window_points": bres["window_points"],
+                       "stride_points_per_slide": bres["stride_points_per_slide"], "path": "batched"},
+            "vs_plain": bres["speedup"],
+            "slides_per_s": bres["batched_slides_per_s"],
+            "p50_ms": bres["batched_step_p50_ms"], "p99_ms": bres["batched_step_p99_ms"],
+            "exact_labels": {"batched_vs_plain": bres["labels_equal_plain"],
+                             "transfer_inclusive_vs_plain": bres["transfer_inclusive"]["labels_equal_plain"]},
+            "transfer_inclusive": dict(bres["transfer_inclusive"]),
+            "roofline": roofline_obj(B, avg_ms, "k_count_points (64-stream launch)", "k_count_points_64streams"),
+            "batched": bres,
+        })
+        log("batched: %s" % {k: v for k, v in bres.items() if k not in ("phase_ms",)})
+        log("roofline: %s" % out["roofline"])
+    if (not only or only == "plain") and args.plain:
+        out["plain"] = {}
+        for name in args.plain.split(","):
+            log("plain leg: %s, %d timed slides" % (name, args.plain_steps))
+            steps = args.plain_steps if name != "skewed3d" else max(5, args.plain_steps // 3)
+            out["plain"][name] = plain_leg(R, WL, name, steps, args.warmup + 1, autotune)
+            log("plain %s: %s" % (name, out["plain"][name]))
+    if (not only or only == "cpu") and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(R, WL, args.cpu_seconds, args.bf_seconds)
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    if (not only or only == "served") and not args.no_serve:
         log("served leg: %d streams x %d slides" % (args.serve_streams, args.serve_slides))
         out["served"] = served_leg(R, WL, args.serve_streams, args.serve_slides,
                                    [float(x) for x in args.serve_rates.split(",")], 64, 2, 64)

@@ -209,6 +209,18 @@ int rtdbs_server_get_stats(rtdbs_server *srv, rtdbs_server_stats *stats);
 int rtdbs_server_reset_stats(rtdbs_server *srv);
 void rtdbs_server_destroy(rtdbs_server *srv);
 
+/* ---- pinned host memory --------------------------------------------------
+ * Page-locked host memory. The host-buffer slides (rtdbs_stream_slide,
+ * rtdbs_batch_slide, rtdbs_server_submit) copy strides and outputs that lie in
+ * such memory by DMA directly, as the reference does with its pinned
+ * h_label / h_cluster_id (dbscan.cpp:54-55); other host buffers go through the
+ * engine's own pinned staging (one extra host memcpy each way). A batch's
+ * copies merge when consecutive members' buffers abut: strides one after the
+ * other, and each member's label then cid, members in batch order. */
+int rtdbs_host_alloc(size_t bytes, void **ptr);
+/* Frees memory from rtdbs_host_alloc (any other pointer: RTDBS_EINVAL). */
+int rtdbs_host_free(void *ptr);
+
 /* ---- device memory helpers (so callers need no other GPU runtime) -------- */
 int rtdbs_device_alloc(size_t bytes, void **ptr);
 int rtdbs_device_free(void *ptr);

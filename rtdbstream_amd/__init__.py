@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "RtdbsError", "Params", "Stream", "Batch", "Server", "DeviceBuffer", "lib", "library_path",
+    "RtdbsError", "Params", "Stream", "Batch", "Server", "DeviceBuffer", "HostBuffer", "lib", "library_path",
     "LABEL_CORE", "LABEL_BORDER", "LABEL_NOISE", "device_count", "autotune",
 ]
 
@@ -130,6 +130,8 @@ SIGNATURES = [
     ("rtdbs_server_get_stats", ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
     ("rtdbs_server_reset_stats", ctypes.c_int, [_P]),
     ("rtdbs_server_destroy", None, [_P]),
+    ("rtdbs_host_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    ("rtdbs_host_free", ctypes.c_int, [_P]),
     ("rtdbs_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     ("rtdbs_device_free", ctypes.c_int, [_P]),
     ("rtdbs_memcpy_h2d", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
@@ -241,6 +243,21 @@ def _f32(a, n, dim, what):
     return a
 
 
+def _out_i32(a, n, what):
+    """A caller-supplied output array (the engine writes n int32 into it), or
+    a new one. Anything else - another dtype or size, a strided view, a
+    read-only array - is refused: the library copies n*4 bytes into it."""
+    if a is None:
+        return np.empty(n, np.int32)
+    if not isinstance(a, np.ndarray):
+        raise ValueError("%s must be a numpy int32 array of %d elements" % (what, n))
+    if a.dtype != np.int32 or a.size != n:
+        raise ValueError("%s must be int32 with %d elements (got %s with %d)" % (what, n, a.dtype, a.size))
+    if not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError("%s must be C-contiguous and writeable" % what)
+    return a
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -279,6 +296,39 @@ class DeviceBuffer:
             pass
 
 
+class HostBuffer:
+    """Pinned host memory from the engine (rtdbs_host_alloc). Host-buffer
+    slides copy arrays that live here by DMA, without the engine's staging
+    copy; `array` returns numpy views into it."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().rtdbs_host_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self._raw = (ctypes.c_char * self.nbytes).from_address(self.ptr)
+
+    def array(self, dtype, count, offset=0):
+        """A numpy view of `count` elements of `dtype` at byte `offset`."""
+        dt = np.dtype(dtype)
+        if offset < 0 or offset + dt.itemsize * count > self.nbytes:
+            raise ValueError("view [%d, %d) outside the %d-byte buffer" % (offset, offset + dt.itemsize * count,
+                                                                            self.nbytes))
+        return np.frombuffer(self._raw, dtype=dt, count=count, offset=offset)
+
+    def free(self):
+        if self.ptr:
+            self._raw = None
+            lib().rtdbs_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     """One sliding window (the reference's ScanState + slide loop body)."""
 
@@ -308,10 +358,8 @@ class Stream:
 
     def slide(self, stride_points, label=None, cid=None):
         pts = _f32(stride_points, self.S, self.dim, "stride")
-        if label is None:
-            label = np.empty(self.W, np.int32)
-        if cid is None:
-            cid = np.empty(self.W, np.int32)
+        label = _out_i32(label, self.W, "label")
+        cid = _out_i32(cid, self.W, "cid")
         _check(lib().rtdbs_stream_slide(self.h, _ptr(pts), _ptr(label), _ptr(cid)))
         return label, cid
 
@@ -383,11 +431,17 @@ class Batch:
 
     def slide(self, strides, labels=None, cids=None):
         n = len(self.streams)
+        if len(strides) != n:
+            raise ValueError("one stride per stream: %d streams, %d strides" % (n, len(strides)))
         strides = [_f32(x, s.S, s.dim, "stride") for x, s in zip(strides, self.streams)]
         if labels is None:
-            labels = [np.empty(s.W, np.int32) for s in self.streams]
+            labels = [None] * n
         if cids is None:
-            cids = [np.empty(s.W, np.int32) for s in self.streams]
+            cids = [None] * n
+        if len(labels) != n or len(cids) != n:
+            raise ValueError("one label and one cid array per stream")
+        labels = [_out_i32(x, s.W, "label") for x, s in zip(labels, self.streams)]
+        cids = [_out_i32(x, s.W, "cid") for x, s in zip(cids, self.streams)]
         A = (ctypes.c_void_p * n)(*[a.ctypes.data for a in strides])
         Lb = (ctypes.c_void_p * n)(*[a.ctypes.data for a in labels])
         C = (ctypes.c_void_p * n)(*[a.ctypes.data for a in cids])
@@ -467,10 +521,8 @@ class Server:
     def submit(self, sid, stride_points, label=None, cid=None):
         dim, W, S = self._meta[sid]
         pts = _f32(stride_points, S, dim, "stride")
-        if label is None:
-            label = np.empty(W, np.int32)
-        if cid is None:
-            cid = np.empty(W, np.int32)
+        label = _out_i32(label, W, "label")
+        cid = _out_i32(cid, W, "cid")
         t = ctypes.c_int64()
         _check(lib().rtdbs_server_submit(self.h, int(sid), _ptr(pts), _ptr(label), _ptr(cid), ctypes.byref(t)))
         self._keep[t.value] = (pts, label, cid)

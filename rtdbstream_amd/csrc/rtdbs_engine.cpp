@@ -161,8 +161,10 @@ static Layout make_layout(const rtdbs_params &p, int R) {
     L.off_tile_a = take(ntile * 4);
     L.off_tile_b = take(ntile * 4);
     L.off_in = take(W * d * 4);
-    L.off_label = take(W * 4);
-    L.off_cid = take(W * 4);
+    // label and cid adjacent: a host-buffer slide copies both down at once
+    // when the caller's arrays are adjacent too
+    L.off_label = take(W * 8);
+    L.off_cid = L.off_label + W * 4;
     L.total = o;
     return L;
 }
@@ -379,14 +381,7 @@ extern "C" int rtdbs_stream_slide(rtdbs_stream *s, const float *stride_points, i
     if (!stride_points || !label || !cid) return fail(RTDBS_EINVAL, "NULL buffer");
     if (!s->initialized) return fail(RTDBS_ESTATE, "slide before rtdbs_stream_init");
     HIPCHK(hipSetDevice(s->device));
-    rtdbs_batch *b = s->solo;
-    const size_t S = (size_t)s->p.stride, W = (size_t)s->p.window, d = (size_t)s->p.dim;
-    HIPCHK(hipMemcpyAsync(s->d_in, stride_points, S * d * 4, hipMemcpyHostToDevice, b->hs));
-    int rc = batch_launch_solo(b, s->d_in, s->d_label, s->d_cid);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(label, s->d_label, W * 4, hipMemcpyDeviceToHost, b->hs));
-    HIPCHK(hipMemcpyAsync(cid, s->d_cid, W * 4, hipMemcpyDeviceToHost, b->hs));
-    return batch_sync(b);
+    return batch_slide_host(s->solo, &stride_points, &label, &cid);
 }
 
 extern "C" int rtdbs_stream_slide_async(rtdbs_stream *s, const float *stride_points_dev, int32_t *label_dev,
@@ -484,6 +479,8 @@ static int batch_alloc(rtdbs_batch *b) {
     b->d_pre = (int32_t *)(b->dmem + desc_b + io_b);
     b->d_stats = (unsigned long long *)(b->dmem + desc_b + io_b + pre_b);
     HIPCHK(hipHostMalloc((void **)&b->h_io, (size_t)n * sizeof(IO), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&b->h_desc, (size_t)n * sizeof(StreamDev), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&b->h_pre, (size_t)4 * (n + 1) * 4, hipHostMallocDefault));
     HIPCHK(hipStreamCreateWithFlags(&b->hs, hipStreamNonBlocking));
     return RTDBS_OK;
 }
@@ -516,6 +513,7 @@ int batch_set_members(rtdbs_batch *b, rtdbs_stream *const *streams, int n) {
     b->gens.assign(n, 0);
     b->stage_io.resize(n);
     for (int i = 0; i < n; i++) b->stage_io[i] = IO{streams[i]->d_in, streams[i]->d_label, streams[i]->d_cid};
+    b->xfer_ready = false;  // host-buffer transfer layout follows the members
     return refresh_batch(b);
 }
 
@@ -528,6 +526,10 @@ void batch_free(rtdbs_batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->hs) (void)hipStreamDestroy(b->hs);
     if (b->h_io) (void)hipHostFree(b->h_io);
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    if (b->h_pre) (void)hipHostFree(b->h_pre);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->d_xfer) (void)hipFree(b->d_xfer);
     if (b->dmem) (void)hipFree(b->dmem);
     delete b;
 }
@@ -553,9 +555,12 @@ static int waves_per_cu() { return env_int("RTDBS_WAVES_PER_CU", 16, 1, 64); }
 // the batch is built and whenever a member stream was re-initialised.
 int refresh_batch(rtdbs_batch *b) {
     const int n = b->n;
+    // the device may still read the tables (and the copy from their pinned
+    // mirrors may be pending)
     if (b->hs) HIPCHK(hipStreamSynchronize(b->hs));
-    std::vector<StreamDev> desc(n);
-    std::vector<int32_t> pre(4 * (n + 1), 0);
+    StreamDev *desc = b->h_desc;
+    int32_t *pre = b->h_pre;
+    std::memset(pre, 0, (size_t)4 * (n + 1) * 4);
     long long rows_total = 0, cells_total = 0;
     for (int i = 0; i < n; i++) {
         rtdbs_stream *s = b->streams[i];
@@ -572,8 +577,10 @@ int refresh_batch(rtdbs_batch *b) {
         cells_total += s->p.window;
         b->gens[i] = s->gen;
     }
-    HIPCHK(hipMemcpy(b->d_desc, desc.data(), (size_t)n * sizeof(StreamDev), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->d_pre, pre.data(), pre.size() * 4, hipMemcpyHostToDevice));
+    // asynchronous uploads from the pinned mirrors, ordered before the next
+    // launch on the batch's stream
+    HIPCHK(hipMemcpyAsync(b->d_desc, desc, (size_t)n * sizeof(StreamDev), hipMemcpyHostToDevice, b->hs));
+    HIPCHK(hipMemcpyAsync(b->d_pre, pre, (size_t)4 * (n + 1) * 4, hipMemcpyHostToDevice, b->hs));
     b->shape.stiles = pre[0 * (n + 1) + n];
     b->shape.wtiles = pre[1 * (n + 1) + n];
     b->shape.chunks = pre[2 * (n + 1) + n];
@@ -764,6 +771,131 @@ int batch_launch(rtdbs_batch *b, const IO *io) {
     return RTDBS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// host-buffer slides
+
+struct Copy {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
+// Appends a copy, merged into the previous one when both ranges abut.
+static void add_copy(std::vector<Copy> &v, void *dst, const void *src, size_t bytes) {
+    if (!v.empty()) {
+        Copy &c = v.back();
+        if ((char *)c.dst + c.bytes == (char *)dst && (const char *)c.src + c.bytes == (const char *)src) {
+            c.bytes += bytes;
+            return;
+        }
+    }
+    v.push_back(Copy{dst, src, bytes});
+}
+
+// Offsets of every member's stride and outputs in the transfer area and the
+// pinned staging (same layout); (re)allocates both when they are too small.
+static int ensure_xfer(rtdbs_batch *b) {
+    if (b->xfer_ready) return RTDBS_OK;
+    const int n = b->n;
+    b->in_off.assign(n, 0);
+    b->out_off.assign(n, 0);
+    size_t o = 0;
+    for (int i = 0; i < n; i++) {
+        const rtdbs_stream *s = b->streams[i];
+        b->in_off[i] = o;
+        o += (size_t)s->p.stride * s->p.dim * 4;
+    }
+    o = (o + 255) & ~size_t(255);
+    for (int i = 0; i < n; i++) {
+        b->out_off[i] = o;
+        o += (size_t)b->streams[i]->p.window * 8;
+    }
+    const size_t total = o;
+    // grown geometrically: a server lane's members change with every batch
+    const size_t grow = std::max(total, total + total / 2);
+    if (!b->solo && total > b->d_xfer_bytes) {
+        HIPCHK(hipStreamSynchronize(b->hs));
+        const size_t want = std::max(grow, b->d_xfer_bytes * 2);
+        if (b->d_xfer) (void)hipFree(b->d_xfer);
+        b->d_xfer = nullptr;
+        b->d_xfer_bytes = 0;
+        hipError_t e = hipMalloc((void **)&b->d_xfer, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(RTDBS_ENOMEM, "transfer area of %zu bytes: %s", want, hipGetErrorString(e));
+        }
+        b->d_xfer_bytes = want;
+    }
+    if (total > b->h_stage_bytes) {
+        HIPCHK(hipStreamSynchronize(b->hs));
+        if (b->h_stage) (void)hipHostFree(b->h_stage);
+        b->h_stage = nullptr;
+        b->h_stage_bytes = 0;
+        const size_t want = b->solo ? total : grow;
+        hipError_t e = hipHostMalloc((void **)&b->h_stage, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(RTDBS_ENOMEM, "pinned staging of %zu bytes: %s", want, hipGetErrorString(e));
+        }
+        b->h_stage_bytes = want;
+    }
+    if (!b->solo) {
+        for (int i = 0; i < n; i++) {
+            const size_t W = (size_t)b->streams[i]->p.window;
+            b->stage_io[i] = IO{(const float *)(b->d_xfer + b->in_off[i]), (int32_t *)(b->d_xfer + b->out_off[i]),
+                                (int32_t *)(b->d_xfer + b->out_off[i] + W * 4)};
+        }
+    }
+    b->xfer_ready = true;
+    return RTDBS_OK;
+}
+
+int batch_slide_host(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids) {
+    int rc = batch_check_current(b);
+    if (rc) return rc;
+    rc = ensure_xfer(b);
+    if (rc) return rc;
+    std::vector<Copy> up, down, post;
+    for (int i = 0; i < b->n; i++) {
+        const rtdbs_stream *s = b->streams[i];
+        const size_t nb = (size_t)s->p.stride * s->p.dim * 4;
+        void *dev = (void *)b->stage_io[i].in;
+        if (host_pinned(in[i], nb)) {
+            add_copy(up, dev, in[i], nb);
+        } else {
+            char *h = b->h_stage + b->in_off[i];
+            std::memcpy(h, in[i], nb);
+            add_copy(up, dev, h, nb);
+        }
+    }
+    for (const Copy &c : up) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, b->hs));
+    if (b->solo)
+        rc = batch_launch_solo(b, b->stage_io[0].in, b->stage_io[0].label, b->stage_io[0].cid);
+    else
+        rc = batch_launch(b, b->stage_io.data());
+    if (rc) return rc;
+    for (int i = 0; i < b->n; i++) {
+        const size_t nb = (size_t)b->streams[i]->p.window * 4;
+        const IO &io = b->stage_io[i];
+        int32_t *dst[2] = {labels[i], cids[i]};
+        const int32_t *src[2] = {io.label, io.cid};
+        for (int k = 0; k < 2; k++) {
+            if (host_pinned(dst[k], nb)) {
+                add_copy(down, dst[k], src[k], nb);
+            } else {
+                char *h = b->h_stage + b->out_off[i] + (size_t)k * nb;
+                add_copy(down, h, src[k], nb);
+                post.push_back(Copy{dst[k], h, nb});
+            }
+        }
+    }
+    for (const Copy &c : down) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, b->hs));
+    rc = batch_sync(b);
+    if (rc) return rc;
+    for (const Copy &c : post) std::memcpy(c.dst, c.src, c.bytes);
+    return RTDBS_OK;
+}
+
 }  // namespace rtdbs
 
 extern "C" int rtdbs_batch_create(rtdbs_stream *const *streams, int n, int use_graph, rtdbs_batch **out) {
@@ -793,19 +925,7 @@ extern "C" int rtdbs_batch_slide(rtdbs_batch *b, const float *const *stride_poin
     if (!b || !stride_points || !labels || !cids) return fail(RTDBS_EINVAL, "NULL argument");
     for (int i = 0; i < b->n; i++)
         if (!stride_points[i] || !labels[i] || !cids[i]) return fail(RTDBS_EINVAL, "NULL buffer for stream %d", i);
-    for (int i = 0; i < b->n; i++) {
-        rtdbs_stream *s = b->streams[i];
-        HIPCHK(hipMemcpyAsync(s->d_in, stride_points[i], (size_t)s->p.stride * s->p.dim * 4,
-                              hipMemcpyHostToDevice, b->hs));
-    }
-    int rc = batch_launch(b, b->stage_io.data());
-    if (rc) return rc;
-    for (int i = 0; i < b->n; i++) {
-        rtdbs_stream *s = b->streams[i];
-        HIPCHK(hipMemcpyAsync(labels[i], s->d_label, (size_t)s->p.window * 4, hipMemcpyDeviceToHost, b->hs));
-        HIPCHK(hipMemcpyAsync(cids[i], s->d_cid, (size_t)s->p.window * 4, hipMemcpyDeviceToHost, b->hs));
-    }
-    return batch_sync(b);
+    return batch_slide_host(b, stride_points, labels, cids);
 }
 
 extern "C" int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *stride_points_dev,
@@ -944,6 +1064,52 @@ extern "C" int rtdbs_stream_get_timing(rtdbs_stream *s, double *ms, int64_t *lau
     int rc = ensure_solo(s);
     if (rc) return rc;
     return rtdbs_batch_get_timing(s->solo, ms, launches, max_phases, names, names_len);
+}
+
+// ---------------------------------------------------------------------------
+// pinned host memory
+
+#include <map>
+
+static std::mutex g_pin_mu;
+static std::map<uintptr_t, size_t> g_pinned;  // start -> bytes of each rtdbs_host_alloc block
+
+namespace rtdbs {
+bool host_pinned(const void *p, size_t bytes) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+}  // namespace rtdbs
+
+extern "C" int rtdbs_host_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(RTDBS_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(RTDBS_ENOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pinned[(uintptr_t)*ptr] = bytes;
+    return RTDBS_OK;
+}
+
+extern "C" int rtdbs_host_free(void *ptr) {
+    if (!ptr) return RTDBS_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        auto it = g_pinned.find((uintptr_t)ptr);
+        if (it == g_pinned.end()) return fail(RTDBS_EINVAL, "pointer was not returned by rtdbs_host_alloc");
+        g_pinned.erase(it);
+    }
+    HIPCHK(hipHostFree(ptr));
+    return RTDBS_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -51,6 +51,20 @@ struct rtdbs_batch {
     int32_t *d_cell_pre = nullptr;
     rtdbs::IO *h_io = nullptr;  // pinned mirror of d_io
     bool io_valid = false;
+    rtdbs::StreamDev *h_desc = nullptr;  // pinned mirrors of the descriptor and prefix tables
+    int32_t *h_pre = nullptr;            // (uploaded asynchronously on the batch's stream)
+    // Host-buffer slides. The device side of each member's I/O (stage_io): the
+    // stream's own arena buffers in a solo batch, else one contiguous transfer
+    // area holding every member's stride (in_off), then its label and cid
+    // next to each other (out_off), so that copies of host buffers laid out the
+    // same way merge into one. h_stage is pinned staging in the same layout for
+    // host buffers that are not engine-pinned memory.
+    bool xfer_ready = false;
+    char *d_xfer = nullptr;
+    size_t d_xfer_bytes = 0;
+    char *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    std::vector<size_t> in_off, out_off;
     rtdbs::BatchArgs args{};
     rtdbs::LaunchShape shape{};
     hipStream_t hs = nullptr;
@@ -78,6 +92,13 @@ int refresh_batch(rtdbs_batch *b);
 int batch_launch_solo(rtdbs_batch *b, const float *in, int32_t *label, int32_t *cid);
 int batch_launch(rtdbs_batch *b, const IO *io);
 int batch_sync(rtdbs_batch *b);
+// One slide of every member with host buffers (synchronous): strides up,
+// slide, labels and ids down, each set of copies merged where the ranges abut.
+// Buffers in engine-pinned memory (rtdbs_host_alloc) are copied by DMA
+// directly, others through the batch's pinned staging.
+int batch_slide_host(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids);
+// True if [p, p + bytes) lies inside one rtdbs_host_alloc allocation.
+bool host_pinned(const void *p, size_t bytes);
 int batch_collect_timing(rtdbs_batch *b);
 void batch_free(rtdbs_batch *b);
 // Allocates a batch able to hold up to `capacity` streams (members set later).

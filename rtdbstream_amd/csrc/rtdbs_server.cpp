@@ -112,6 +112,8 @@ void lane_main(rtdbs_server *srv, int lane) {
     rtdbs_batch *b = srv->lanes[lane];
     std::vector<rtdbs_stream *> members;
     std::vector<Request *> reqs;
+    std::vector<const float *> ins;
+    std::vector<int32_t *> labels, cids;
     while (true) {
         {
             std::unique_lock<std::mutex> lk(srv->mu);
@@ -132,22 +134,19 @@ void lane_main(rtdbs_server *srv, int lane) {
             for (Request *r : reqs) members.push_back(srv->streams[r->sid]->s);
         }
         int rc = batch_set_members(b, members.data(), (int)members.size());
-        for (size_t i = 0; i < reqs.size() && rc == RTDBS_OK; i++) {
-            rtdbs_stream *s = members[i];
-            hipError_t e = hipMemcpyAsync(s->d_in, reqs[i]->in, (size_t)s->p.stride * s->p.dim * 4,
-                                          hipMemcpyHostToDevice, b->hs);
-            if (e != hipSuccess) rc = fail(RTDBS_EHIP, "stride upload failed: %s", hipGetErrorString(e));
+        if (rc == RTDBS_OK) {
+            ins.clear();
+            labels.clear();
+            cids.clear();
+            for (Request *r : reqs) {
+                ins.push_back(r->in);
+                labels.push_back(r->label);
+                cids.push_back(r->cid);
+            }
+            // strides up, one launch per phase, labels down (merged copies;
+            // pinned staging unless the caller's buffers are engine-pinned)
+            rc = batch_slide_host(b, ins.data(), labels.data(), cids.data());
         }
-        if (rc == RTDBS_OK) rc = batch_launch(b, b->stage_io.data());
-        for (size_t i = 0; i < reqs.size() && rc == RTDBS_OK; i++) {
-            rtdbs_stream *s = members[i];
-            hipError_t e = hipMemcpyAsync(reqs[i]->label, s->d_label, (size_t)s->p.window * 4,
-                                          hipMemcpyDeviceToHost, b->hs);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(reqs[i]->cid, s->d_cid, (size_t)s->p.window * 4, hipMemcpyDeviceToHost, b->hs);
-            if (e != hipSuccess) rc = fail(RTDBS_EHIP, "label download failed: %s", hipGetErrorString(e));
-        }
-        if (rc == RTDBS_OK) rc = batch_sync(b);
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(srv->mu);
@@ -272,10 +271,12 @@ extern "C" int rtdbs_server_close_stream(rtdbs_server *srv, int32_t stream_id) {
         auto it = srv->streams.find(stream_id);
         if (it == srv->streams.end()) return fail(RTDBS_ENOTFOUND, "unknown stream %d", stream_id);
         ss = it->second;
+        if (ss->closing) return fail(RTDBS_ENOTFOUND, "stream %d is being closed", stream_id);
         ss->closing = true;
         cancel_queued(srv, ss);
+        // a lane, an evict or a readmit may hold the stream
         srv->cv_done.wait(lk, [&] { return !ss->busy; });
-        srv->streams.erase(it);
+        srv->streams.erase(stream_id);
         if (ss->s) srv->bytes_in_use -= ss->bytes;
     }
     srv->cv_done.notify_all();
@@ -294,9 +295,11 @@ extern "C" int rtdbs_server_evict_stream(rtdbs_server *srv, int32_t stream_id) {
         auto it = srv->streams.find(stream_id);
         if (it == srv->streams.end()) return fail(RTDBS_ENOTFOUND, "unknown stream %d", stream_id);
         ss = it->second;
-        if (!ss->s) return RTDBS_OK;  // already evicted
+        if (ss->closing) return fail(RTDBS_ENOTFOUND, "stream %d is being closed", stream_id);
         srv->cv_done.wait(lk, [&] { return !ss->busy; });
-        ss->busy = true;  // keep lanes away while checkpointing
+        if (ss->closing) return fail(RTDBS_ENOTFOUND, "stream %d is being closed", stream_id);
+        if (!ss->s) return RTDBS_OK;  // already evicted
+        ss->busy = true;  // keeps lanes, close, evict and readmit away while checkpointing
         s = ss->s;
     }
     std::vector<float> win((size_t)s->p.window * s->p.dim);
@@ -313,6 +316,7 @@ extern "C" int rtdbs_server_evict_stream(rtdbs_server *srv, int32_t stream_id) {
         }
     }
     if (rc == RTDBS_OK) rtdbs_stream_destroy(s);
+    srv->cv_done.notify_all();  // close / evict / readmit wait for !busy
     srv->cv_budget.notify_all();
     srv->cv_work.notify_all();
     return rc;
@@ -326,9 +330,20 @@ extern "C" int rtdbs_server_readmit_stream(rtdbs_server *srv, int32_t stream_id)
         auto it = srv->streams.find(stream_id);
         if (it == srv->streams.end()) return fail(RTDBS_ENOTFOUND, "unknown stream %d", stream_id);
         ss = it->second;
+        if (ss->closing) return fail(RTDBS_ENOTFOUND, "stream %d is being closed", stream_id);
+        srv->cv_done.wait(lk, [&] { return !ss->busy; });
+        if (ss->closing) return fail(RTDBS_ENOTFOUND, "stream %d is being closed", stream_id);
         if (ss->s) return RTDBS_OK;
+        // held from here on: a close waits for the rebuild, a concurrent
+        // readmit or evict waits and then sees the rebuilt stream
+        ss->busy = true;
         int rc = admit_bytes(srv, lk, ss->bytes);
-        if (rc) return rc;
+        if (rc) {
+            ss->busy = false;
+            lk.unlock();
+            srv->cv_done.notify_all();
+            return rc;
+        }
     }
     rtdbs_stream *s = nullptr;
     int rc = rtdbs_stream_create(&ss->params, &s);
@@ -336,17 +351,23 @@ extern "C" int rtdbs_server_readmit_stream(rtdbs_server *srv, int32_t stream_id)
     if (rc) {
         std::string msg = rtdbs_last_error();
         if (s) rtdbs_stream_destroy(s);
-        std::lock_guard<std::mutex> lk(srv->mu);
-        srv->bytes_in_use -= ss->bytes;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->bytes_in_use -= ss->bytes;
+            ss->busy = false;
+        }
+        srv->cv_done.notify_all();
         srv->cv_budget.notify_all();
         return fail(rc, "%s", msg.c_str());
     }
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         ss->s = s;
+        ss->busy = false;
         ss->ckpt.clear();
         ss->ckpt.shrink_to_fit();
     }
+    srv->cv_done.notify_all();
     srv->cv_work.notify_all();
     return RTDBS_OK;
 }
