@@ -161,6 +161,39 @@ class PinnedSlides:
         self.hout.free()
 
 
+def batched_only(R, WL, n_streams, steps, warmup):
+    """Only the batched graph replays of the 64-stream workload (default
+    grid and kernel forms, no trial slides, no plain legs): a clean launch
+    sequence for kernel traces and counter passes."""
+    specs = WL.multi_stream_specs(n_streams=n_streams, slides=warmup + steps)
+    datas = [WL.stream_points(sp) for sp in specs]
+    bufs = []
+    for sp, d in zip(specs, datas):
+        db = R.DeviceBuffer(d.nbytes)
+        db.upload(d)
+        bufs.append((db, R.DeviceBuffer(sp["window"] * 4), R.DeviceBuffer(sp["window"] * 4)))
+    streams = []
+    for sp, d in zip(specs, datas):
+        s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+        s.init(d[:sp["window"]])
+        streams.append(s)
+    b = R.Batch(streams, use_graph=True)
+    a0 = None
+    for t in range(warmup + steps):
+        if t == warmup:
+            R.lib().rtdbs_device_synchronize()
+            a0 = time.perf_counter()
+        b.slide_async([bufs[i][0].addr((sp["window"] + t * sp["stride"]) * sp["dim"] * 4)
+                       for i, sp in enumerate(specs)], [x[1].addr() for x in bufs], [x[2].addr() for x in bufs])
+        b.sync()
+    el = time.perf_counter() - a0
+    b.close()
+    for s in streams:
+        s.close()
+    return {"streams": len(specs), "steps": steps, "ms_per_slide": round(el / steps * 1e3, 4),
+            "points_per_s": round(sum(sp["stride"] for sp in specs) * steps / el, 1)}
+
+
 def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
     """The 64-stream workload: plain (one stream at a time) and batched (one
     graph replay per slide), each device-resident and transfer-inclusive."""
@@ -447,40 +480,74 @@ def cpu_baseline(R, WL, seconds, bf_seconds):
     return out
 
 
-def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, seed=7):
+def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, seed=7, data_slides=10,
+               swap_frac=0.5, swap_factor=1.0):
     """Request server under Poisson arrivals at `factors` x the serial plain
-    capacity: throughput and p50/p99 slide latency (submit -> labels in host
-    memory). Each stream's last result is compared with its plain result."""
-    specs = WL.served_stream_specs(n_streams=n_streams, slides=slides)
+    capacity: throughput and p50/p99 slide latency (submit -> labels and ids
+    in the client's pinned host buffers). Each stream holds W + data_slides*S
+    points; slide t inserts stride t mod data_slides (a full slide of work;
+    the windows repeat with that period). The sweep stops one rate after the
+    server falls behind the offered rate (its capacity). Every stream's last
+    result is compared with its plain result."""
+    specs = WL.served_stream_specs(n_streams=n_streams, slides=data_slides)
     datas = [WL.stream_points(sp) for sp in specs]
+    # client buffers in pinned memory: every stream's strides, and one label
+    # + cid pair per stream that its requests (run in order) write in turn
+    in_b = sum(sp["stride"] * sp["dim"] * 4 * data_slides for sp in specs)
+    out_b = sum(2 * sp["window"] * 4 for sp in specs)
+    hin, hout = R.HostBuffer(in_b), R.HostBuffer(out_b)
+    strides, labs, cids = [], [], []
+    oi = oo = 0
+    for sp, d in zip(specs, datas):
+        W, S, dim = sp["window"], sp["stride"], sp["dim"]
+        row = []
+        for t in range(data_slides):
+            a = hin.array(np.float32, S * dim, offset=oi)
+            a[:] = d[W + t * S: W + (t + 1) * S].reshape(-1)
+            row.append(a)
+            oi += S * dim * 4
+        strides.append(row)
+        labs.append(hout.array(np.int32, W, offset=oo))
+        cids.append(hout.array(np.int32, W, offset=oo + W * 4))
+        oo += 2 * W * 4
 
     def stride(i, t):
-        sp = specs[i]
-        return datas[i][sp["window"] + t * sp["stride"]: sp["window"] + (t + 1) * sp["stride"]]
+        return strides[i][t % data_slides]
 
-    # serial plain slides on the same GPU (host buffers, like the server)
+    # serial plain slides on the same GPU, same buffers
     plain_ms, last = [], []
     for i, sp in enumerate(specs):
         s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
         s.init(datas[i][:sp["window"]])
         for t in range(slides):
             a = time.perf_counter()
-            l, c = s.slide(stride(i, t))
+            s.slide(stride(i, t), labs[i], cids[i])
             plain_ms.append((time.perf_counter() - a) * 1e3)
-        last.append((l.copy(), c.copy()))
+        last.append((labs[i].copy(), cids[i].copy()))
         s.close()
     plain_total_s = sum(plain_ms) / 1e3
     capacity = len(plain_ms) / plain_total_s
+    stream_bytes = sum(R.Stream.device_bytes(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"])
+                       for sp in specs)
     out = {"streams": n_streams, "slides_per_stream": slides,
            "window_range": [min(sp["window"] for sp in specs), max(sp["window"] for sp in specs)],
+           "window_points": sum(sp["window"] for sp in specs), "stream_device_gb": round(stream_bytes / 2**30, 2),
            "hbm_budget_gb": budget_gb, "max_batch": max_batch, "lanes": lanes,
+           "client_buffers": "pinned (rtdbs_host_alloc)",
            "plain_serial_slides_per_s": round(capacity, 1),
            "plain_serial_points_per_s": round(sum(sp["stride"] for sp in specs) * slides / plain_total_s, 1),
            "plain_p50_ms": pct(plain_ms, 50), "plain_p99_ms": pct(plain_ms, 99), "sweep": []}
     rng = np.random.default_rng(seed)
     exact = True
-    for f in factors:
-        srv = R.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_batch=max_batch, lanes=lanes)
+    behind = 0
+    # the sweep at the full budget, then one run whose budget holds only
+    # swap_frac of the streams' device bytes (streams swap through host memory)
+    runs = [(f, int(budget_gb * (1 << 30)), False) for f in factors] + [(swap_factor, int(swap_frac * stream_bytes),
+                                                                          True)]
+    for f, budget, swap in runs:
+        if not swap and behind >= 2:
+            continue
+        srv = R.Server(hbm_budget_bytes=budget, max_batch=max_batch, lanes=lanes, swap=swap)
         sids = [srv.open_stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"],
                                 datas[i][:sp["window"]]) for i, sp in enumerate(specs)]
         order = np.repeat(np.arange(n_streams), slides)
@@ -500,25 +567,37 @@ def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, s
                     time.sleep((due - now) * 0.5)
             t = nxt[i]
             nxt[i] += 1
-            tickets.append((i, t, srv.submit(sids[i], stride(i, t))))
-        res_last = {}
-        for i, t, tk in tickets:
-            l, c = srv.wait(tk, 120000)
-            if t == slides - 1:
-                res_last[i] = (l, c)
+            tickets.append(srv.submit(sids[i], stride(i, t), labs[i], cids[i]))
+        for tk in tickets:
+            srv.wait(tk, 120000)
         wall = time.perf_counter() - t0
         st = srv.stats()
         srv.close()
         for i in range(n_streams):
-            exact = exact and np.array_equal(res_last[i][0], last[i][0]) and np.array_equal(res_last[i][1], last[i][1])
+            exact = exact and np.array_equal(labs[i], last[i][0]) and np.array_equal(cids[i], last[i][1])
         nreq = len(order)
-        out["sweep"].append({"rate_factor": f, "offered_slides_per_s": round(f * capacity, 1),
-                             "slides_per_s": round(nreq / wall, 1),
-                             "points_per_s": round(sum(specs[i]["stride"] for i in order) / wall, 1),
-                             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                             "mean_batch": round(st["mean_batch"], 2), "batches": st["batches"]})
-        log("served f=%.2f: %s" % (f, out["sweep"][-1]))
+        rate = nreq / wall
+        row = {"rate_factor": f, "offered_slides_per_s": round(f * capacity, 1), "slides_per_s": round(rate, 1),
+               "points_per_s": round(sum(specs[i]["stride"] for i in order) / wall, 1),
+               "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+               "mean_batch": round(st["mean_batch"], 2), "batches": st["batches"]}
+        if swap:
+            row.update({"budget_gb": round(budget / 2**30, 3), "swap_ins": st["swap_ins"],
+                        "swap_outs": st["swap_outs"], "failed": st["failed"]})
+            out["swap_run"] = row
+            log("served with swapping (budget %.2f of the streams' bytes): %s" % (swap_frac, row))
+            continue
+        out["sweep"].append(row)
+        log("served f=%.2f: %s" % (f, row))
+        if rate < 0.9 * f * capacity:
+            behind += 1
+    sat = max(out["sweep"], key=lambda r: r["slides_per_s"])
+    out["capacity_slides_per_s"] = sat["slides_per_s"]
+    out["capacity_points_per_s"] = sat["points_per_s"]
+    out["capacity_vs_plain"] = round(sat["slides_per_s"] / capacity, 3)
     out["labels_equal_plain"] = bool(exact)
+    hin.free()
+    hout.free()
     return out
 
 
@@ -535,11 +614,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-streams", type=int, default=256)
-    ap.add_argument("--serve-slides", type=int, default=3)
-    ap.add_argument("--serve-rates", default="0.5,1,2,4")
+    ap.add_argument("--serve-slides", type=int, default=20, help="requests per served stream")
+    ap.add_argument("--serve-rates", default="0.5,1,2,4,8,16,32",
+                    help="offered rates as multiples of the serial plain capacity (stops past saturation)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this size (profiler byte-counter calibration)")
-    ap.add_argument("--only", default="", help="run only this leg (batched, plain, cpu, served) - for profiling")
+    ap.add_argument("--only", default="",
+                    help="run only this leg (batched, plain, cpu, served; batched-kernels: the 64-stream graph "
+                         "replays alone, for profiling)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default grid side and kernel form instead of the autotuner's choice")
     args = ap.parse_args()
@@ -564,6 +646,10 @@ def main():
     only = args.only
     autotune = not args.no_autotune
     out = {"metric": METRIC}
+    if only == "batched-kernels":
+        out["batched_only"] = batched_only(R, WL, args.batch_streams, args.steps, args.warmup)
+        print(json.dumps(out), flush=True)
+        return
     if not only or only == "batched":
         log("batched leg: %d streams, %d warmup + %d timed slides" % (args.batch_streams, args.warmup, args.steps))
         bres = batched_leg(R, WL, args.batch_streams, args.steps, args.warmup, autotune)

@@ -167,7 +167,10 @@ typedef struct rtdbs_server_config {
     int32_t lanes;           /* batches that may be on the device at once (HIP streams) */
     int32_t use_graph;       /* capture recurring batch shapes as HIP graphs */
     int32_t admit_wait_ms;   /* open_stream waits this long for budget to free up */
-    int32_t reserved[4];
+    int32_t swap;            /* 1: streams beyond the budget live as host checkpoints and are
+                                swapped in when a request of theirs is ready, least recently
+                                used idle streams out (open_stream then never waits) */
+    int32_t reserved[3];
 } rtdbs_server_config;
 
 typedef struct rtdbs_server_stats {
@@ -180,12 +183,17 @@ typedef struct rtdbs_server_stats {
     double max_ms;
     size_t bytes_in_use;
     int32_t streams;
-    int32_t reserved;
+    int32_t resident;      /* streams whose state is on the device */
+    int64_t swap_ins;      /* streams rebuilt from a checkpoint (readmits included) */
+    int64_t swap_outs;     /* streams checkpointed to host memory (evictions included) */
 } rtdbs_server_stats;
 
 int rtdbs_server_create(const rtdbs_server_config *cfg, rtdbs_server **out);
 /* Admits a stream (and loads its first window, host memory) if its device
- * bytes fit the budget, else RTDBS_EBUDGET. */
+ * bytes fit the budget, else RTDBS_EBUDGET. With cfg.swap, a stream that does
+ * not fit now is opened as a host checkpoint of its first window and swapped
+ * in when its first request is ready (RTDBS_EBUDGET only if it exceeds the
+ * whole budget). */
 int rtdbs_server_open_stream(rtdbs_server *srv, const rtdbs_params *params, const float *first_window,
                              int32_t *stream_id);
 /* Releases the stream's device memory. Pending requests are cancelled. */
@@ -288,7 +296,7 @@ typedef struct {
     int64_t border_tests;  /* distance tests of the border pass */
     int64_t unites;        /* union-find unite calls */
     int64_t merge_pairs;   /* touching element pairs found by the merge */
-    int32_t cell_parallel; /* 1: count and border ran cell-parallel; 0: point-parallel (large
+    int32_t cell_parallel; /* 1: count and border ran cell-parallel; 0: either ran point-parallel (large
                               batches). The merge counters come from the cell-parallel merge,
                               which both paths use by default */
     int32_t pad;

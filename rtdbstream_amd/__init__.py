@@ -67,7 +67,8 @@ class StreamInfo(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [
         ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int32), ("lanes", ctypes.c_int32),
-        ("use_graph", ctypes.c_int32), ("admit_wait_ms", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("use_graph", ctypes.c_int32), ("admit_wait_ms", ctypes.c_int32), ("swap", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -76,7 +77,7 @@ class ServerStats(ctypes.Structure):
         ("completed", ctypes.c_int64), ("failed", ctypes.c_int64), ("batches", ctypes.c_int64),
         ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
         ("max_ms", ctypes.c_double), ("bytes_in_use", ctypes.c_size_t), ("streams", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("resident", ctypes.c_int32), ("swap_ins", ctypes.c_int64), ("swap_outs", ctypes.c_int64),
     ]
 
 
@@ -487,13 +488,14 @@ class Batch:
 class Server:
     """Continuous-batching slide server with HBM-budget admission."""
 
-    def __init__(self, hbm_budget_bytes, max_batch=64, lanes=2, use_graph=False, admit_wait_ms=0):
+    def __init__(self, hbm_budget_bytes, max_batch=64, lanes=2, use_graph=False, admit_wait_ms=0, swap=False):
         cfg = ServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_batch = int(max_batch)
         cfg.lanes = int(lanes)
         cfg.use_graph = int(use_graph)
         cfg.admit_wait_ms = int(admit_wait_ms)
+        cfg.swap = int(bool(swap))
         h = ctypes.c_void_p()
         _check(lib().rtdbs_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h.value
@@ -555,7 +557,7 @@ class Server:
     def stats(self):
         s = ServerStats()
         _check(lib().rtdbs_server_get_stats(self.h, ctypes.byref(s)))
-        return {k: getattr(s, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
 
     def reset_stats(self):
         _check(lib().rtdbs_server_reset_stats(self.h))

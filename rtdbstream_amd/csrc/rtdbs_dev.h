@@ -86,7 +86,6 @@ struct StreamDev {
     // clustering
     uint8_t *flags;   // W: kFlagCore | kFlagDense
     int32_t *parent;  // W: union-find over positions, root = minimum slot
-    int32_t *canon;   // W: canonical id (min core slot) of core positions
     int32_t *rep;     // W: union-find element of each position (cell-parallel path)
     int32_t *tile_a;  // ceil(W/kWTile): survivors per tile
     int32_t *tile_b;  // ceil(W/kWTile): cell starts per tile
@@ -118,7 +117,8 @@ struct LaunchShape {
     int32_t lpq;             // lanes per query of the point-parallel count (1, 2 or 4)
     int32_t lpq_border;      // lanes per query of the point-parallel output kernel
     int32_t cellpar_blocks;  // blocks of the cell-parallel count and border kernels
-    int32_t cellpar;         // 1: cell-parallel count/border, 0: point-parallel
+    int32_t cellpar;         // 1: cell-parallel output kernel, 0: point-parallel
+    int32_t count_cell;      // 1: cell-parallel range count (writes fcore and rep), 0: point-parallel
     int32_t union_cells;     // point-parallel path: 1 cell-level merge kernels, 0 the cell-parallel merge
     int32_t rank_split;      // 1: stride rank with 64-key blocks (small strides), 0: 256-key blocks
 };
@@ -131,7 +131,6 @@ enum Phase {
     kPhRows,
     kPhCount,
     kPhUnion,
-    kPhFlatten,
     kPhBorder,
     kNumPhases
 };

@@ -116,7 +116,7 @@ int validate_params(const rtdbs_params *p) {
 struct Layout {
     size_t off_mut, off_skey, off_sx, off_sy, off_sz, off_akey, off_ax, off_ay, off_az;
     size_t off_bkey0, off_bx0, off_by0, off_bz0, off_bkey1, off_bx1, off_by1, off_bz1;
-    size_t off_cstart, off_ckey, off_pcell, off_rows, off_cflag, off_fcore, off_flags, off_parent, off_canon, off_rep,
+    size_t off_cstart, off_ckey, off_pcell, off_rows, off_cflag, off_fcore, off_flags, off_parent, off_rep,
         off_tile_a, off_tile_b, off_in, off_label, off_cid;
     size_t total;
 };
@@ -156,7 +156,6 @@ static Layout make_layout(const rtdbs_params &p, int R) {
     L.off_fcore = take(W * 4);
     L.off_flags = take(W);
     L.off_parent = take(W * 4);
-    L.off_canon = take(W * 4);
     L.off_rep = take(W * 4);
     L.off_tile_a = take(ntile * 4);
     L.off_tile_b = take(ntile * 4);
@@ -269,7 +268,6 @@ extern "C" int rtdbs_stream_create(const rtdbs_params *params, rtdbs_stream **ou
     D.fcore = (int32_t *)(A + L.off_fcore);
     D.flags = (uint8_t *)(A + L.off_flags);
     D.parent = (int32_t *)(A + L.off_parent);
-    D.canon = (int32_t *)(A + L.off_canon);
     D.rep = (int32_t *)(A + L.off_rep);
     D.tile_a = (int32_t *)(A + L.off_tile_a);
     D.tile_b = (int32_t *)(A + L.off_tile_b);
@@ -605,6 +603,10 @@ int refresh_batch(rtdbs_batch *b) {
     const int pmode = n == 1 ? b->streams[0]->p.neighbor_mode : 0;  // a single stream's own setting
     const int mode = nmode ? nmode : pmode;
     b->shape.cellpar = mode == 1 || (mode == 0 && cells_total <= kCellParMaxPoints);
+    // the range count's own form (RTDBS_COUNT_MODE: 1 cell-parallel, 2
+    // point-parallel; otherwise as the output kernel)
+    const int cmode = env_int("RTDBS_COUNT_MODE", 0, 0, 2);
+    b->shape.count_cell = cmode ? (cmode == 1) : b->shape.cellpar;
     // the cell-parallel merge also wins in large batches (RTDBS_UNION_CELLS=1
     // selects the older cell-level merge kernels there)
     b->shape.union_cells = env_int("RTDBS_UNION_CELLS", 0, 0, 1);
@@ -987,7 +989,7 @@ extern "C" int rtdbs_batch_get_stats(rtdbs_batch *b, int index, rtdbs_stats *out
     out->border_tests = (int64_t)v[kStatBorderTests];
     out->unites = (int64_t)v[kStatUnites];
     out->merge_pairs = (int64_t)v[kStatMergePairs];
-    out->cell_parallel = b->shape.cellpar;
+    out->cell_parallel = b->shape.cellpar && b->shape.count_cell;
     return RTDBS_OK;
 }
 

@@ -29,8 +29,7 @@
 namespace rtdbs {
 
 const char *const kPhaseNames[kNumPhases] = {"sort_expire", "rank_compact", "merge_final",
-                                             "cells", "rows", "count", "union",
-                                             "flatten", "border"};
+                                             "cells", "rows", "count", "union", "border"};
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -1157,7 +1156,7 @@ struct CellView {
     gptr<const uint8_t> cflag;
     gptr<const int32_t> pcell;
     gptr<uint8_t> flags;
-    gptr<int32_t> fcore, parent, canon, rep;
+    gptr<int32_t> fcore, parent, rep;
     int R, dim, min_pts;
     uint32_t smask;
     float eps2;
@@ -1176,7 +1175,6 @@ __device__ __forceinline__ CellView cell_view(const StreamDev &S) {
     v.flags = gl(ldc(S.flags));
     v.fcore = gl(ldc(S.fcore));
     v.parent = gl(ldc(S.parent));
-    v.canon = gl(ldc(S.canon));
     v.rep = gl(ldc(S.rep));
     v.R = ldc(S.R);
     v.dim = ldc(S.dim);
@@ -1363,26 +1361,51 @@ __device__ __forceinline__ int walk_chunk(RowWalk &w, int jb, int *end) {
     return base + min(j, *end - 1);
 }
 
+// A cell's header as each lane loaded it: run start and end, class, and
+// stencil row `lane` (lanes < R). One round trip, issued ahead of its use.
+struct CellHdr {
+    int csv, cev, cfv;
+    uint64_t rw;
+};
+
+__device__ __forceinline__ CellHdr load_hdr(const StreamDev &S, int A) {
+    const gptr<const int32_t> cstart = gl((const int32_t *)ldc(S.cstart));
+    const gptr<const uint8_t> cflag = gl((const uint8_t *)ldc(S.cflag));
+    const gptr<const uint64_t> rows = gl((const uint64_t *)ldc(S.rows));
+    const int R = ldc(S.R);
+    CellHdr h;
+    h.csv = cstart[A];
+    h.cev = cstart[A + 1];
+    h.cfv = cflag[A];
+    // every lane loads (no branch around the load); lanes >= R are cleared
+    // where the row is used
+    h.rw = rows[(size_t)A * R + min(lane_id(), R - 1)];
+    return h;
+}
+
 // Range count = core identification (the reference's __raygen__grid +
 // __intersection__grid, dbscan.cu:155-194,250-270, and set_label,
-// aabb.cu:575-587). Dense cells are core without counting (all pairs within
-// eps). Otherwise each query counts its neighbours, itself included, and
-// stops once the count reaches min_pts. Loads are issued in three rounds -
-// the cell's header and rows, its queries, each candidate chunk - each
-// waited for as a whole.
-template <class ST>
-__device__ void count_cell(const CellView &V, int A, ST &st) {
+// aabb.cu:575-587), for cell A whose header h was loaded ahead. Dense cells
+// are core without counting (all pairs within eps). A cell whose whole
+// stencil holds fewer than min_pts points cannot hold a core: its points are
+// non-core without a distance test. Otherwise each query counts its
+// neighbours, itself included, and stops once the count reaches min_pts.
+// The queries and the first candidate chunk load in one round, after which
+// `prefetch` issues the next cell's header, so that it arrives while this
+// cell computes; each further chunk is a round of its own. `prefetch` runs
+// exactly once on every path.
+template <class ST, class Pre>
+__device__ __forceinline__ void count_cell(const CellView &V, int A, const CellHdr &h, ST &st, Pre &&prefetch) {
     const int lane = lane_id();
     const int dim = V.dim, minpts = V.min_pts;
     const float e2 = V.eps2;
-    const int csv = V.cstart[A], cev = V.cstart[A + 1], cfv = V.cflag[A];
-    const uint64_t rw = row_word(V, A);
-    arrived(rw);
-    const int cs = uni(csv), nA = uni(cev) - cs;
-    const int cf = uni(cfv);
+    const int cs = uni(h.csv), nA = uni(h.cev) - cs;
+    const int cf = uni(h.cfv);
     if (cf & kCellDense) {
+        const int rsv = vslot(V, cs);
+        prefetch();
         st.add(kStatDensePoints, nA);
-        const int rs = uni(vslot(V, cs));
+        const int rs = uni(rsv);
         for (int i = lane; i < nA; i += 64) {
             V.flags[cs + i] = kFlagCore | kFlagDense;
             V.rep[cs + i] = rs;
@@ -1391,26 +1414,41 @@ __device__ void count_cell(const CellView &V, int A, ST &st) {
         return;
     }
     const bool safe = cf & kCellSafe;
-    const RowWalk w0 = row_walk(rw, INT_MAX);
-    int fc = kNoCore, fslot = 0;
+    const RowWalk w0 = row_walk(lane < V.R ? h.rw : 0ull, INT_MAX);
     st.add(kStatQueries, nA);
-    for (int q0 = 0; q0 < nA; q0 += 64) {
+    if (uni(wave_sum(w0.len)) < minpts) {
+        // fewer candidates than min_pts: no core (rep is read for cores only)
+        prefetch();
+        for (int i = lane; i < nA; i += 64) V.flags[cs + i] = 0;
+        if (lane == 0) V.fcore[A] = kNoCore;
+        return;
+    }
+    int fc = kNoCore, fslot = 0;
+    // one block of up to 64 queries; the first block issues the prefetch
+    auto block = [&](const int q0, const bool first) __attribute__((always_inline)) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
         const float qx = V.sx[pq];
         const float qy = dim > 1 ? V.sy[pq] : 0.f;
         const float qz = dim > 2 ? V.sz[pq] : 0.f;
         const int qslot = vslot(V, pq);
+        RowWalk w = w0;
+        int end;
+        int pos = walk_chunk(w, 0, &end);
+        const float cx = V.sx[pos];
+        const float cy = dim > 1 ? V.sy[pos] : 0.f;
+        const float cz = dim > 2 ? V.sz[pos] : 0.f;
+        if (first) prefetch();
+        arrived(qx);
+        arrived(qy);
+        arrived(qz);
+        arrived(cx);
+        arrived(cy);
+        arrived(cz);
         int mine = 0;
         uint64_t open = nq == 64 ? ~0ull : ((1ull << nq) - 1ull);
-        RowWalk w = w0;
-        for (int jb = 0; open; jb += 64) {
-            int end;
-            const int pos = walk_chunk(w, jb, &end);
-            if (jb >= end) break;
-            const float cx = V.sx[pos];
-            const float cy = dim > 1 ? V.sy[pos] : 0.f;
-            const float cz = dim > 2 ? V.sz[pos] : 0.f;
+        // the open queries against candidate chunk [jb, jb + 64)
+        auto test = [&](const int jb, const float x, const float y, const float z) __attribute__((always_inline)) {
             const bool v = jb + lane < end;
             uint64_t m = open;
             st.add(kStatCountTests, (unsigned long long)__popcll(m) * (unsigned)min(64, end - jb));
@@ -1418,10 +1456,20 @@ __device__ void count_cell(const CellView &V, int A, ST &st) {
                 const int qi = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const int c =
-                    __popcll(__ballot(v & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2)));
+                    __popcll(__ballot(v & (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), x, y, z) < e2)));
                 if (lane == qi) mine += c;
             }
             open &= ~__ballot(mine >= minpts);
+        };
+        // the first chunk outside the loop: its wait leaves the prefetch in flight
+        test(0, cx, cy, cz);
+        for (int jb = 64; open; jb += 64) {
+            pos = walk_chunk(w, jb, &end);
+            if (jb >= end) break;
+            const float x = V.sx[pos];
+            const float y = dim > 1 ? V.sy[pos] : 0.f;
+            const float z = dim > 2 ? V.sz[pos] : 0.f;
+            test(jb, x, y, z);
         }
         if (lane < nq) {
             V.flags[cs + q0 + lane] = mine >= minpts ? kFlagCore : 0;
@@ -1433,24 +1481,46 @@ __device__ void count_cell(const CellView &V, int A, ST &st) {
             fc = cs + q0 + l;
             fslot = lane_i(qslot, l);
         }
-    }
+    };
+    block(0, true);
+    for (int q0 = 64; q0 < nA; q0 += 64) block(q0, false);
     if (lane == 0) V.fcore[A] = fc;  // first (minimum-slot) core: the cell's representative
     // safe cell: one union-find element, the representative's slot
     if (safe)
         for (int i = lane; i < nA; i += 64) V.rep[cs + i] = fc != kNoCore ? fslot : 0;
 }
 
+// One wave per cell in a grid-stride walk over all cells of the launch; the
+// next cell's header is in flight while the current cell computes.
 template <class ST>
-__global__ __launch_bounds__(64 * kCellWaves) void k_count(BatchArgs a) {
+__global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_count(BatchArgs a) {
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     ST st;
-    int s = 0;  // stream of the wave's current cell (cells rise, so streams do)
-    for (int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6)); gc < total; gc += nw) {
-        s = uni(find_stream_from(a.cell_pre, a.n, gc, s));
-        const int c = gc - uni(ldc(a.cell_pre[s]));
-        stats_stream(a, st, s);
-        count_cell(cell_view(a.s[s]), c, st);
+    int gc = uni(blockIdx.x * kCellWaves + (threadIdx.x >> 6));
+    if (gc < total) {
+        int s = uni(find_stream_from(a.cell_pre, a.n, gc, 0));  // cells rise, so streams do
+        int c = gc - uni(ldc(a.cell_pre[s]));
+        CellHdr h = load_hdr(a.s[s], c);
+        while (true) {
+            const int gn = gc + nw;
+            int sn = s, cn = 0;
+            CellHdr hn = h;
+            // (past the last cell it reloads the last one: no branch around the loads)
+            auto prefetch = [&]() {
+                const int gq = min(gn, total - 1);
+                sn = uni(find_stream_from(a.cell_pre, a.n, gq, s));
+                cn = gq - uni(ldc(a.cell_pre[sn]));
+                hn = load_hdr(a.s[sn], cn);
+            };
+            stats_stream(a, st, s);
+            count_cell(cell_view(a.s[s]), c, h, st, prefetch);
+            if (gn >= total) break;
+            gc = gn;
+            s = sn;
+            c = cn;
+            h = hn;
+        }
     }
     stats_flush(a, st);
 }
@@ -1678,6 +1748,10 @@ __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &
     eb.n += nf;
 }
 
+// Candidate positions before which a safe cell's merge goes over candidate
+// cells first (below it, scanning the positions costs fewer round trips).
+constexpr int kCellLevelMin = 192;
+
 // Cluster merge (the reference's __raygen__cluster + unite,
 // dbscan.cu:279-321,354-394,9-43), cell-parallel: one wave per cell A with
 // cores. Its candidates are the stencil positions before A's own first
@@ -1688,6 +1762,15 @@ __device__ __forceinline__ void unite_once_per_rep(const BatchArgs &a, EdgeBuf &
 // of any core of A joins it. The cores of an unsafe cell are separate
 // elements, and the cell's own earlier points are candidates too. A
 // candidate's element is its safe cell's representative or its own slot.
+//
+// A safe cell with many candidates (dense regions) goes over its candidate
+// cells first, 64 per round: one safe candidate cell joins A as soon as its
+// first core is within eps of a core of A - one test per cell pair instead of
+// a scan of the cell's points, which is all the merge needs (the analogue of
+// the reference's dense-cell spheres, dbscan.cu:378-393). Only the candidate
+// cells not settled that way (unsafe cells, whose cores are separate
+// elements, and safe cells whose first core is out of reach) have their
+// points scanned.
 template <class ST>
 __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const CellHead &h, ST &st, EdgeBuf &eb) {
     const int lane = lane_id();
@@ -1701,6 +1784,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
     const uint64_t rw = row_word(V, A);
     const uint64_t rk = V.skey[h.fc];  // the representative's slot
     RowWalk w0;
+    bool by_cells = false;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1717,71 +1801,125 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
             arrived(rw);
             w0 = row_walk(rw, safeA ? cs : cs + nA);
             if (w0.rest == 0ull) return;
+            by_cells = safeA && uni(wave_sum(w0.len)) >= kCellLevelMin;
         }
         const bool qcore = lane < nq && (denseA || (qf & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
         const int qslot = (int)((uint32_t)qk & V.smask);
         const int repA = uni((int)((uint32_t)rk & V.smask));
-        RowWalk w = w0;
-        for (int jb = 0;; jb += 64) {
-            int end;
-            const int pos = walk_chunk(w, jb, &end);
-            if (jb >= end) break;
-            const int f = V.flags[pos];
-            const float cx = V.sx[pos];
-            const float cy = dim > 1 ? V.sy[pos] : 0.f;
-            const float cz = dim > 2 ? V.sz[pos] : 0.f;
-            const int rp = V.rep[pos];  // meaningful for cores only
-            arrived(cx);
-            arrived(cy);
-            arrived(cz);
-            arrived(rp);
-            const bool bcore = jb + lane < end && (f & kFlagCore);
-            const uint64_t bm = __ballot(bcore);
-            if (bm == 0ull) continue;
-            const int rep = bcore ? rp : 0;
-            const unsigned chunk = (unsigned)min(64, end - jb);
-            uint64_t m = qm;
-            if (safeA) {
-                // lanes holding one element form a run (a safe cell's points are
-                // adjacent); one touching lane settles its whole run
-                const int prev_rep = __shfl_up(rep, 1);  // every lane takes part in the shuffles
-                const int prev_core = __shfl_up((int)bcore, 1);
-                const uint64_t starts = __ballot(lane == 0 || prev_rep != rep || !prev_core);
-                const uint64_t after = starts & ~((2ull << lane) - 1ull);  // run starts above this lane
-                const uint64_t upto = after ? (after & (~after + 1ull)) - 1ull : ~0ull;
-                const uint64_t below = starts & ((2ull << lane) - 1ull);
-                const uint64_t from = ~((1ull << (63 - __clzll((long long)below))) - 1ull);
-                const uint64_t run = upto & from;  // this lane's run
-                // touched candidates as one wave mask: the loop carries a scalar,
-                // not a lane predicate
-                uint64_t t = 0;
-                while (m) {
-                    const int qi = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    st.add(kStatMergeTests, chunk);
-                    // every lane evaluates the test, no branch per query
-                    t |= __ballot(sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2) & bm;
-                    if ((__ballot((t & run) != 0ull) | ~bm) == ~0ull) break;  // every candidate run settled
-                }
-                unite_once_per_rep(a, eb, s, (t >> lane) & 1ull, rep, repA, st);
-            } else {
-                while (m) {
-                    const int qi = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    st.add(kStatMergeTests, chunk);
-                    const bool hit = bcore & (pos < cs + q0 + qi) &
-                                     (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
-                    unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
+        // The cores qm of this block against the candidate positions of walk w.
+        auto scan = [&](RowWalk w) __attribute__((always_inline)) {
+            for (int jb = 0;; jb += 64) {
+                int end;
+                const int pos = walk_chunk(w, jb, &end);
+                if (jb >= end) break;
+                const int f = V.flags[pos];
+                const float cx = V.sx[pos];
+                const float cy = dim > 1 ? V.sy[pos] : 0.f;
+                const float cz = dim > 2 ? V.sz[pos] : 0.f;
+                const int rp = V.rep[pos];  // meaningful for cores only
+                arrived(cx);
+                arrived(cy);
+                arrived(cz);
+                arrived(rp);
+                const bool bcore = jb + lane < end && (f & kFlagCore);
+                const uint64_t bm = __ballot(bcore);
+                if (bm == 0ull) continue;
+                const int rep = bcore ? rp : 0;
+                const unsigned chunk = (unsigned)min(64, end - jb);
+                uint64_t m = qm;
+                if (safeA) {
+                    // lanes holding one element form a run (a safe cell's points are
+                    // adjacent); one touching lane settles its whole run
+                    const int prev_rep = __shfl_up(rep, 1);  // every lane takes part in the shuffles
+                    const int prev_core = __shfl_up((int)bcore, 1);
+                    const uint64_t starts = __ballot(lane == 0 || prev_rep != rep || !prev_core);
+                    const uint64_t after = starts & ~((2ull << lane) - 1ull);  // run starts above this lane
+                    const uint64_t upto = after ? (after & (~after + 1ull)) - 1ull : ~0ull;
+                    const uint64_t below = starts & ((2ull << lane) - 1ull);
+                    const uint64_t from = ~((1ull << (63 - __clzll((long long)below))) - 1ull);
+                    const uint64_t run = upto & from;  // this lane's run
+                    // touched candidates as one wave mask: the loop carries a scalar,
+                    // not a lane predicate
+                    uint64_t t = 0;
+                    while (m) {
+                        const int qi = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        st.add(kStatMergeTests, chunk);
+                        // every lane evaluates the test, no branch per query
+                        t |= __ballot(sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2) & bm;
+                        if ((__ballot((t & run) != 0ull) | ~bm) == ~0ull) break;  // every candidate run settled
+                    }
+                    unite_once_per_rep(a, eb, s, (t >> lane) & 1ull, rep, repA, st);
+                } else {
+                    while (m) {
+                        const int qi = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        st.add(kStatMergeTests, chunk);
+                        const bool hit = bcore & (pos < cs + q0 + qi) &
+                                         (sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), cx, cy, cz) < e2);
+                        unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
+                    }
                 }
             }
+        };
+        if (!by_cells) {
+            scan(w0);
+            continue;
+        }
+        // candidate cells: row r (lane r) covers positions [rx, rx + len), all
+        // before cs, i.e. the cells pcell[rx] .. pcell[rx + len - 1] (< A)
+        const int clo = w0.len > 0 ? V.pcell[w0.rx] : 0;
+        const int chi = w0.len > 0 ? V.pcell[w0.rx + w0.len - 1] + 1 : 0;
+        RowWalk cw;
+        cw.rx = clo;
+        cw.len = chi - clo;
+        cw.rest = __ballot(cw.len > 0);
+        for (int jb = 0;; jb += 64) {
+            int endc;
+            const int B = walk_chunk(cw, jb, &endc);
+            if (jb >= endc) break;
+            const bool valid = jb + lane < endc;
+            const int fb = V.fcore[B];
+            const int cfb = V.cflag[B];
+            const int bs = V.cstart[B];
+            const int be = V.cstart[B + 1];
+            arrived(fb);
+            arrived(be);
+            const bool has = valid && fb != kNoCore;
+            const int fp = has ? fb : cs;  // a valid position either way
+            const float bx = V.sx[fp];
+            const float by = dim > 1 ? V.sy[fp] : 0.f;
+            const float bz = dim > 2 ? V.sz[fp] : 0.f;
+            const uint64_t bk = V.skey[fp];
+            arrived(bx);
+            arrived(by);
+            arrived(bz);
+            arrived(bk);
+            // safe candidate cells: A's cores against each cell's first core
+            const uint64_t cand = __ballot(has && (cfb & kCellSafe));
+            uint64_t hit = 0;
+            for (uint64_t m = qm; m && hit != cand;) {
+                const int qi = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                st.add(kStatMergeTests, (unsigned)__popcll(cand & ~hit));
+                hit |= __ballot(sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), bx, by, bz) < e2) & cand;
+            }
+            // a safe cell's element is its first core's slot; one lane per cell
+            unite_once_per_rep(a, eb, s, (hit >> lane) & 1ull, (int)((uint32_t)bk & V.smask), repA, st);
+            // the cells with cores not settled: scan their points
+            RowWalk pw;
+            pw.rx = bs;
+            pw.len = (has && !((hit >> lane) & 1ull)) ? be - bs : 0;
+            pw.rest = __ballot(pw.len > 0);
+            if (pw.rest) scan(pw);
         }
     }
 }
 
 template <class ST>
-__global__ __launch_bounds__(64 * kCellWaves) void k_union(BatchArgs a) {
+__global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_union(BatchArgs a) {
     __shared__ Edge edges[kCellWaves][kEdgeCap];
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
@@ -1864,12 +2002,11 @@ __global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
 }
 
 // Border / noise, point-parallel form (large batches): LPQ lanes per
-// non-core point. With ROOTS (rep[] holds every position's element) the
-// kernel also writes the cores' outputs, takes candidate roots from the
-// union-find itself and advances each stream's ring cursor, as the
-// cell-parallel output kernel does; without it (the cell-level merge ran) it
-// reads the canonical ids that k_flatten wrote.
-template <int LPQ, bool ROOTS>
+// non-core point, which takes the minimum root among the cores within eps.
+// The kernel also writes the cores' outputs (the root of each core's
+// union-find element, found here: no separate flatten pass) and advances each
+// stream's ring cursor, as the cell-parallel output kernel does.
+template <int LPQ>
 __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
     const int s = uni(ql.x.s);
@@ -1884,7 +2021,7 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     const float qy = dim > 1 ? V.sy[pp] : 0.f;
     const float qz = dim > 2 ? V.sz[pp] : 0.f;
     const uint64_t qk = V.skey[pp];
-    const int qr = ROOTS ? V.rep[pp] : 0;
+    const int qr = V.rep[pp];
     arrived(c);
     arrived(qx);
     arrived(qy);
@@ -1893,18 +2030,16 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     arrived(qr);
     const int32_t *parent = ldc(a.s[s].parent);
     const IO io = a.use_io1 ? a.io1 : a.io[s];
-    if (ROOTS) {
-        if (vb == ldc(a.pre_chunk[s]) * LPQ && lane_id() == 0) {  // the stream's first block
-            StreamMut *m = ldc(a.s[s].mut);
-            if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
-            m->t = m->t + 1;
-            m->u = m->u + 1 == ldc(a.s[s].units) ? 0 : m->u + 1;
-        }
-        if (p < ql.x.p1 && (qf & kFlagCore) && ql.sub == 0) {
-            const int slot = (int)((uint32_t)qk & V.smask);
-            gl(io.label)[slot] = 0;
-            gl(io.cid)[slot] = uf_root(parent, qr);
-        }
+    if (vb == ldc(a.pre_chunk[s]) * LPQ && lane_id() == 0) {  // the stream's first block
+        StreamMut *m = ldc(a.s[s].mut);
+        if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
+        m->t = m->t + 1;
+        m->u = m->u + 1 == ldc(a.s[s].units) ? 0 : m->u + 1;
+    }
+    if (p < ql.x.p1 && (qf & kFlagCore) && ql.sub == 0) {
+        const int slot = (int)((uint32_t)qk & V.smask);
+        gl(io.label)[slot] = 0;
+        gl(io.cid)[slot] = uf_root(parent, qr);
     }
     const bool mine = p < ql.x.p1 && !(qf & kFlagCore);
     const float e2 = V.eps2;
@@ -1916,7 +2051,7 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
             [&](int j, float cx, float cy, float cz) {
                 tests++;
                 if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (V.flags[j] & kFlagCore))
-                    best = min(best, ROOTS ? uf_root(parent, V.rep[j]) : V.canon[j]);
+                    best = min(best, uf_root(parent, V.rep[j]));
                 return false;
             },
             ql.sub, LPQ);
@@ -1933,9 +2068,9 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
     }
 }
 
-template <int LPQ, bool ROOTS>
+template <int LPQ>
 __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
-    border_vblock<LPQ, ROOTS>(a, blockIdx.x);
+    border_vblock<LPQ>(a, blockIdx.x);
 }
 
 // First (minimum-slot) core of every cell - its union-find representative -
@@ -2413,39 +2548,6 @@ __device__ __forceinline__ void union_points_chunk(const BatchArgs &a, int g) {
 
 __global__ __launch_bounds__(64) void k_union_points(BatchArgs a) { union_points_chunk(a, blockIdx.x); }
 
-// Canonical ids of cores (post_cluster_t, aabb.cu:173-196): the slot of the
-// root of the core's union-find element - read from rep[] when the merge kept
-// it (use_rep), else derived from the cell. Block 0 of each stream also
-// advances the ring cursor; no later kernel of the slide reads it.
-__device__ __forceinline__ void flatten_chunk(const BatchArgs &a, int g, int use_rep) {
-    const ChunkCtx x = chunk_of(a, g);
-    const int s = uni(x.s);
-    const StreamDev &S = a.s[s];
-    const IO io = a.use_io1 ? a.io1 : a.io[s];
-    const int p = x.p0 + lane_id();
-    if (g == ldc(a.pre_chunk[s]) && lane_id() == 0) {
-        StreamMut *m = S.mut;
-        if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
-        m->t = m->t + 1;
-        m->u = m->u + 1 == S.units ? 0 : m->u + 1;
-    }
-    if (p >= x.p1) return;
-    const CellView V = cell_view(S);
-    const int f = V.flags[p];
-    const uint64_t k = V.skey[p];
-    const int rp = use_rep ? V.rep[p] : 0;
-    arrived(k);
-    arrived(rp);
-    if (!(f & kFlagCore)) return;
-    const int v = uf_find(ldc(S.parent), use_rep ? rp : rep_slot(S, p));  // minimum core slot of the cluster
-    V.canon[p] = v;
-    const int slot = (int)((uint32_t)k & V.smask);
-    gl(io.label)[slot] = 0;
-    gl(io.cid)[slot] = v;
-}
-
-__global__ __launch_bounds__(64) void k_flatten(BatchArgs a, int use_rep) { flatten_chunk(a, blockIdx.x, use_rep); }
-
 // Plain device copy of 16-byte words: a kernel of known HBM traffic (reads and
 // writes `n16` * 16 bytes) used to scale the profiler's byte counters.
 __global__ __launch_bounds__(kTileThreads) void k_copy16(const int4 *__restrict__ src, int4 *__restrict__ dst,
@@ -2518,46 +2620,38 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
         // their own between it and its neighbours.
         const bool timed = ev && ((phase_mask >> kPhCount) & 1u);
         const hipEvent_t e0 = timed ? ev[2 * kPhCount] : nullptr, e1 = timed ? ev[2 * kPhCount + 1] : nullptr;
-        if (sh.cellpar && a.stats)
+        if (sh.count_cell && a.stats)
             e = launch_timed(k_count<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
-        else if (sh.cellpar) e = launch_timed(k_count<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
+        else if (sh.count_cell)
+            e = launch_timed(k_count<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, st, e0, e1, a);
         else if (sh.lpq >= 4) e = launch_timed(k_count_points<4>, sh.chunks * 4, 64, st, e0, e1, a);
         else if (sh.lpq == 2) e = launch_timed(k_count_points<2>, sh.chunks * 2, 64, st, e0, e1, a);
         else e = launch_timed(k_count_points<1>, sh.chunks, 64, st, e0, e1, a);
         if (e != hipSuccess) return e;
     }
     PH_BEGIN(kPhUnion);
-    if (sh.cellpar) {
-        if (a.stats) RTDBS_LAUNCH(k_union<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
-        else RTDBS_LAUNCH(k_union<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
-    } else if (sh.union_cells) {
-        RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
-        RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
-        RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
-    } else {
-        // point-parallel count, cell-parallel merge
+    // the cell-parallel count wrote every cell's first core and every
+    // position's union-find element; after the point-parallel count two small
+    // kernels do
+    if (!sh.count_cell) {
         RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
         RTDBS_LAUNCH(k_rep, sh.wtiles, kTileThreads, a);
-        if (a.stats) RTDBS_LAUNCH(k_union<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
-        else RTDBS_LAUNCH(k_union<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+    }
+    if (sh.union_cells) {
+        RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
+        RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);
+    } else if (a.stats) {
+        RTDBS_LAUNCH(k_union<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+    } else {
+        RTDBS_LAUNCH(k_union<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     }
     PH_END(kPhUnion);
-    PH_BEGIN(kPhFlatten);
-    // the output kernels find the roots themselves, except after the
-    // cell-level merge kernels (rep[] is not kept then): flatten writes the
-    // cores' canonical ids for them
-    if (!sh.cellpar && sh.union_cells) RTDBS_LAUNCH(k_flatten, sh.chunks, 64, a, 0);
-    PH_END(kPhFlatten);
     PH_BEGIN(kPhBorder);
     if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.cellpar) RTDBS_LAUNCH(k_border<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
-    else if (sh.union_cells) {
-        if (sh.lpq_border >= 4) RTDBS_LAUNCH((k_border_points<4, false>), sh.chunks * 4, 64, a);
-        else if (sh.lpq_border == 2) RTDBS_LAUNCH((k_border_points<2, false>), sh.chunks * 2, 64, a);
-        else RTDBS_LAUNCH((k_border_points<1, false>), sh.chunks, 64, a);
-    } else if (sh.lpq_border >= 4) RTDBS_LAUNCH((k_border_points<4, true>), sh.chunks * 4, 64, a);
-    else if (sh.lpq_border == 2) RTDBS_LAUNCH((k_border_points<2, true>), sh.chunks * 2, 64, a);
-    else RTDBS_LAUNCH((k_border_points<1, true>), sh.chunks, 64, a);
+    else if (sh.lpq_border >= 4) RTDBS_LAUNCH(k_border_points<4>, sh.chunks * 4, 64, a);
+    else if (sh.lpq_border == 2) RTDBS_LAUNCH(k_border_points<2>, sh.chunks * 2, 64, a);
+    else RTDBS_LAUNCH(k_border_points<1>, sh.chunks, 64, a);
     PH_END(kPhBorder);
 #undef PH_BEGIN
 #undef PH_END

@@ -53,6 +53,7 @@ struct SStream {
     std::deque<Request *> queue;
     bool busy = false;
     bool closing = false;
+    uint64_t last_used = 0;  // server clock at the last completed slide (or admission)
     // checkpoint while evicted
     std::vector<float> ckpt;
     int64_t ckpt_slides = 0;
@@ -76,8 +77,9 @@ struct rtdbs_server {
     std::vector<std::thread> workers;
     std::vector<rtdbs_batch *> lanes;
     int device = 0;
+    uint64_t clock = 0;  // advanced per completed batch and admission (recency for swapping)
     // stats
-    int64_t completed = 0, failed = 0, batches = 0, batched_streams = 0;
+    int64_t completed = 0, failed = 0, batches = 0, batched_streams = 0, swap_ins = 0, swap_outs = 0;
     std::vector<double> lat_ms;
 };
 
@@ -107,6 +109,95 @@ std::vector<Request *> take_ready(rtdbs_server *srv) {
     return out;
 }
 
+// A queued request whose stream is ready to slide but not on the device
+// (caller holds mu).
+SStream *ready_evicted(rtdbs_server *srv) {
+    for (Request *r : srv->arrivals) {
+        auto it = srv->streams.find(r->sid);
+        if (it == srv->streams.end()) continue;
+        SStream *ss = it->second;
+        if (!ss->s && !ss->busy && !ss->closing && !ss->queue.empty() && ss->queue.front() == r) return ss;
+    }
+    return nullptr;
+}
+
+// Swapping (cfg.swap): brings the stream of the oldest ready request back on
+// the device, checkpointing the least recently used idle streams to host
+// memory until it fits the budget. Victims are destroyed before the stream
+// is rebuilt, so device memory never exceeds the budget; the accounting is
+// reserved up front under the lock, so lanes swapping at the same time never
+// over-admit. Returns true if it swapped a stream in. Caller holds mu via lk.
+bool swap_in_one(rtdbs_server *srv, std::unique_lock<std::mutex> &lk) {
+    SStream *ss = ready_evicted(srv);
+    if (!ss) return false;
+    std::vector<SStream *> victims;
+    size_t freed = 0;
+    while (srv->bytes_in_use - freed + ss->bytes > srv->cfg.hbm_budget_bytes) {
+        SStream *v = nullptr;
+        for (auto &kv : srv->streams) {
+            SStream *c = kv.second;
+            if (!c->s || c->busy || c->closing || !c->queue.empty()) continue;
+            if (!v || c->last_used < v->last_used) v = c;
+        }
+        if (!v) break;
+        v->busy = true;
+        victims.push_back(v);
+        freed += v->bytes;
+    }
+    if (srv->bytes_in_use - freed + ss->bytes > srv->cfg.hbm_budget_bytes) {
+        for (SStream *v : victims) v->busy = false;  // not now: wait for work to finish
+        return false;
+    }
+    ss->busy = true;
+    srv->bytes_in_use = srv->bytes_in_use - freed + ss->bytes;
+    lk.unlock();
+    size_t kept = 0;  // victims whose checkpoint failed stay on the device
+    for (SStream *v : victims) {
+        std::vector<float> win((size_t)v->s->p.window * v->s->p.dim);
+        const int rc = rtdbs_stream_get_window(v->s, win.data());
+        const int64_t slides = v->s->slides;
+        if (rc == RTDBS_OK) rtdbs_stream_destroy(v->s);
+        std::lock_guard<std::mutex> g(srv->mu);
+        if (rc == RTDBS_OK) {
+            v->ckpt.swap(win);
+            v->ckpt_slides = slides;
+            v->s = nullptr;
+            srv->swap_outs++;
+        } else {
+            kept += v->bytes;
+        }
+        v->busy = false;
+    }
+    rtdbs_stream *st = nullptr;
+    int rc = kept ? fail(RTDBS_ENOMEM, "a stream could not be checkpointed to make room") : RTDBS_OK;
+    if (!rc) rc = rtdbs_stream_create(&ss->params, &st);
+    if (!rc) rc = rtdbs_stream_restore(st, ss->ckpt.data(), ss->ckpt_slides);
+    lk.lock();
+    srv->bytes_in_use += kept;
+    if (rc == RTDBS_OK) {
+        ss->s = st;
+        ss->ckpt.clear();
+        ss->ckpt.shrink_to_fit();
+        ss->last_used = ++srv->clock;
+        srv->swap_ins++;
+    } else {
+        if (st) rtdbs_stream_destroy(st);
+        srv->bytes_in_use -= ss->bytes;
+        // the request that asked for the stream fails; later ones retry
+        Request *r = ss->queue.front();
+        ss->queue.pop_front();
+        srv->arrivals.erase(std::remove(srv->arrivals.begin(), srv->arrivals.end(), r), srv->arrivals.end());
+        r->state = kDone;
+        r->status = rc;
+        r->t_done = Clock::now();
+        srv->failed++;
+    }
+    ss->busy = false;
+    srv->cv_done.notify_all();
+    srv->cv_budget.notify_all();
+    return true;
+}
+
 void lane_main(rtdbs_server *srv, int lane) {
     (void)hipSetDevice(srv->device);
     rtdbs_batch *b = srv->lanes[lane];
@@ -121,15 +212,22 @@ void lane_main(rtdbs_server *srv, int lane) {
                 if (srv->stopping) return true;
                 for (Request *r : srv->arrivals) {
                     auto it = srv->streams.find(r->sid);
-                    if (it != srv->streams.end() && !it->second->busy && it->second->s &&
+                    if (it != srv->streams.end() && !it->second->busy && (it->second->s || srv->cfg.swap) &&
                         !it->second->queue.empty() && it->second->queue.front() == r)
                         return true;
                 }
                 return false;
             });
             if (srv->stopping) return;
+            bool swapped = false;
+            if (srv->cfg.swap) swapped = swap_in_one(srv, lk);
+            if (srv->stopping) return;
             reqs = take_ready(srv);
-            if (reqs.empty()) continue;
+            if (reqs.empty()) {
+                // an evicted stream waits for room that only running work frees
+                if (!swapped) srv->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+                continue;
+            }
             members.clear();
             for (Request *r : reqs) members.push_back(srv->streams[r->sid]->s);
         }
@@ -152,10 +250,12 @@ void lane_main(rtdbs_server *srv, int lane) {
             std::lock_guard<std::mutex> lk(srv->mu);
             srv->batches++;
             srv->batched_streams += (int64_t)reqs.size();
+            const uint64_t now_clock = ++srv->clock;
             for (Request *r : reqs) {
                 SStream *ss = srv->streams[r->sid];
                 ss->queue.pop_front();
                 ss->busy = false;
+                ss->last_used = now_clock;
                 r->state = kDone;
                 r->status = rc;
                 r->t_done = now;
@@ -221,7 +321,24 @@ extern "C" int rtdbs_server_open_stream(rtdbs_server *srv, const rtdbs_params *p
     int rc = validate_params(params);
     if (rc) return rc;
     const size_t bytes = stream_device_bytes(*params);
-    {
+    if (srv->cfg.swap) {
+        if (bytes > srv->cfg.hbm_budget_bytes)
+            return fail(RTDBS_EBUDGET, "stream needs %zu bytes, budget is %zu", bytes, srv->cfg.hbm_budget_bytes);
+        std::unique_lock<std::mutex> lk(srv->mu);
+        if (srv->bytes_in_use + bytes > srv->cfg.hbm_budget_bytes) {
+            // no room now: a checkpoint of the first window, swapped in on demand
+            SStream *ss = new SStream();
+            ss->id = srv->next_id++;
+            ss->params = *params;
+            ss->bytes = bytes;
+            ss->ckpt.assign(first_window, first_window + (size_t)params->window * params->dim);
+            ss->ckpt_slides = 0;
+            srv->streams[ss->id] = ss;
+            *stream_id = ss->id;
+            return RTDBS_OK;
+        }
+        srv->bytes_in_use += bytes;
+    } else {
         std::unique_lock<std::mutex> lk(srv->mu);
         rc = admit_bytes(srv, lk, bytes);
         if (rc) return rc;
@@ -243,6 +360,7 @@ extern "C" int rtdbs_server_open_stream(rtdbs_server *srv, const rtdbs_params *p
     ss->params = *params;
     ss->s = s;
     ss->bytes = bytes;
+    ss->last_used = ++srv->clock;
     srv->streams[ss->id] = ss;
     *stream_id = ss->id;
     return RTDBS_OK;
@@ -313,6 +431,7 @@ extern "C" int rtdbs_server_evict_stream(rtdbs_server *srv, int32_t stream_id) {
             ss->ckpt_slides = slides;
             ss->s = nullptr;
             srv->bytes_in_use -= ss->bytes;
+            srv->swap_outs++;
         }
     }
     if (rc == RTDBS_OK) rtdbs_stream_destroy(s);
@@ -364,8 +483,10 @@ extern "C" int rtdbs_server_readmit_stream(rtdbs_server *srv, int32_t stream_id)
         std::lock_guard<std::mutex> lk(srv->mu);
         ss->s = s;
         ss->busy = false;
+        ss->last_used = ++srv->clock;
         ss->ckpt.clear();
         ss->ckpt.shrink_to_fit();
+        srv->swap_ins++;
     }
     srv->cv_done.notify_all();
     srv->cv_work.notify_all();
@@ -465,6 +586,9 @@ extern "C" int rtdbs_server_get_stats(rtdbs_server *srv, rtdbs_server_stats *st)
     st->mean_batch = srv->batches ? (double)srv->batched_streams / (double)srv->batches : 0.0;
     st->bytes_in_use = srv->bytes_in_use;
     st->streams = (int32_t)srv->streams.size();
+    for (auto &kv : srv->streams) st->resident += kv.second->s ? 1 : 0;
+    st->swap_ins = srv->swap_ins;
+    st->swap_outs = srv->swap_outs;
     if (!srv->lat_ms.empty()) {
         std::vector<double> v = srv->lat_ms;
         std::sort(v.begin(), v.end());
@@ -482,7 +606,7 @@ extern "C" int rtdbs_server_get_stats(rtdbs_server *srv, rtdbs_server_stats *st)
 extern "C" int rtdbs_server_reset_stats(rtdbs_server *srv) {
     if (!srv) return fail(RTDBS_EINVAL, "server is NULL");
     std::lock_guard<std::mutex> lk(srv->mu);
-    srv->completed = srv->failed = srv->batches = srv->batched_streams = 0;
+    srv->completed = srv->failed = srv->batches = srv->batched_streams = srv->swap_ins = srv->swap_outs = 0;
     srv->lat_ms.clear();
     return RTDBS_OK;
 }

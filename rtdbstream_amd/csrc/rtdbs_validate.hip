@@ -25,30 +25,46 @@
 #include "rtdbs_host.h"
 
 namespace rtdbs {
+
 namespace {
 
 constexpr int kBfTile = 256;  // candidates staged per LDS tile = threads per block
 
 struct BfArgs {
-    const float *x, *y, *z;
+    const float *pts;  // n points, dim floats each, point-major
     int n, dim, min_pts;
     float e2;
     int32_t *nn, *parent, *label, *cid;
 };
+
+// Coordinates of point i (0 on the axes past dim; those are never read).
+__device__ __forceinline__ void bf_point(const BfArgs &a, int i, float &x, float &y, float &z) {
+    const float *p = a.pts + (size_t)i * a.dim;
+    x = p[0];
+    y = a.dim > 1 ? p[1] : 0.f;
+    z = a.dim > 2 ? p[2] : 0.f;
+}
+
+// Stages candidate j (or zeros past n) into the block's LDS tile.
+__device__ __forceinline__ void bf_stage(const BfArgs &a, int j, float *tx, float *ty, float *tz) {
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (j < a.n) bf_point(a, j, x, y, z);
+    tx[threadIdx.x] = x;
+    ty[threadIdx.x] = y;
+    tz[threadIdx.x] = z;
+}
 
 // Neighbour counts: one query per thread, candidates in LDS tiles.
 __global__ __launch_bounds__(kBfTile) void k_bf_count(BfArgs a) {
     __shared__ float tx[kBfTile], ty[kBfTile], tz[kBfTile];
     const int i = blockIdx.x * kBfTile + threadIdx.x;
     const bool q = i < a.n;
-    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    if (q) bf_point(a, i, qx, qy, qz);
     int cnt = 0;
     for (int t0 = 0; t0 < a.n; t0 += kBfTile) {
         __syncthreads();
-        const int j = t0 + threadIdx.x;
-        tx[threadIdx.x] = j < a.n ? a.x[j] : 0.f;
-        ty[threadIdx.x] = j < a.n ? a.y[j] : 0.f;
-        tz[threadIdx.x] = j < a.n ? a.z[j] : 0.f;
+        bf_stage(a, t0 + threadIdx.x, tx, ty, tz);
         __syncthreads();
         const int m = min(kBfTile, a.n - t0);
         for (int k = 0; k < m; k++) cnt += sqd(a.dim, qx, qy, qz, tx[k], ty[k], tz[k]) < a.e2;
@@ -67,15 +83,14 @@ __global__ __launch_bounds__(kBfTile) void k_bf_union(BfArgs a) {
     __shared__ int tc[kBfTile];
     const int i = blockIdx.x * kBfTile + threadIdx.x;
     const bool q = i < a.n && a.label[i] == 0;
-    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    if (q) bf_point(a, i, qx, qy, qz);
     const int last = min(a.n, (int)(blockIdx.x + 1) * kBfTile);  // candidates below the block's last query
     for (int t0 = 0; t0 < last; t0 += kBfTile) {
         __syncthreads();
         const int j = t0 + threadIdx.x;
         const bool v = j < a.n;
-        tx[threadIdx.x] = v ? a.x[j] : 0.f;
-        ty[threadIdx.x] = v ? a.y[j] : 0.f;
-        tz[threadIdx.x] = v ? a.z[j] : 0.f;
+        bf_stage(a, j, tx, ty, tz);
         tc[threadIdx.x] = v && a.label[j] == 0;
         __syncthreads();
         if (!q) continue;
@@ -97,15 +112,14 @@ __global__ __launch_bounds__(kBfTile) void k_bf_border(BfArgs a) {
     __shared__ int tcid[kBfTile];
     const int i = blockIdx.x * kBfTile + threadIdx.x;
     const bool q = i < a.n && a.label[i] != 0;
-    const float qx = q ? a.x[i] : 0.f, qy = q ? a.y[i] : 0.f, qz = q ? a.z[i] : 0.f;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    if (q) bf_point(a, i, qx, qy, qz);
     int best = INT_MAX;
     for (int t0 = 0; t0 < a.n; t0 += kBfTile) {
         __syncthreads();
         const int j = t0 + threadIdx.x;
         const bool v = j < a.n;
-        tx[threadIdx.x] = v ? a.x[j] : 0.f;
-        ty[threadIdx.x] = v ? a.y[j] : 0.f;
-        tz[threadIdx.x] = v ? a.z[j] : 0.f;
+        bf_stage(a, j, tx, ty, tz);
         tcid[threadIdx.x] = v && a.label[j] == 0 ? a.cid[j] : INT_MAX;
         __syncthreads();
         if (!q) continue;
@@ -117,6 +131,24 @@ __global__ __launch_bounds__(kBfTile) void k_bf_border(BfArgs a) {
         a.label[i] = 1;
         a.cid[i] = best;
     }
+}
+
+// The four launches of one validation, in order, on `st`.
+hipError_t launch_bf(const BfArgs &a, hipStream_t st) {
+    const unsigned blocks = (unsigned)((a.n + kBfTile - 1) / kBfTile);
+    hipLaunchKernelGGL(k_bf_count, dim3(blocks), dim3(kBfTile), 0, st, a);
+    hipLaunchKernelGGL(k_bf_union, dim3(blocks), dim3(kBfTile), 0, st, a);
+    hipLaunchKernelGGL(k_bf_flatten, dim3(blocks), dim3(kBfTile), 0, st, a);
+    hipLaunchKernelGGL(k_bf_border, dim3(blocks), dim3(kBfTile), 0, st, a);
+    return hipGetLastError();
+}
+
+int check_bf_args(int32_t dim, float eps, int32_t min_pts, int64_t n) {
+    if (dim < 1 || dim > 3) return fail(RTDBS_EINVAL, "dim must be 1, 2 or 3 (got %d)", dim);
+    if (!(eps > 0.0f) || !(eps < INFINITY)) return fail(RTDBS_EINVAL, "eps must be finite and > 0");
+    if (min_pts < 1) return fail(RTDBS_EINVAL, "min_pts must be >= 1 (got %d)", min_pts);
+    if (n < 1 || n > (int64_t)INT_MAX - kBfTile) return fail(RTDBS_EINVAL, "n out of range (got %lld)", (long long)n);
+    return RTDBS_OK;
 }
 
 #define BFCHK(expr)                                                                  \
@@ -135,24 +167,15 @@ using namespace rtdbs;
 
 extern "C" int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const float *points, int64_t n,
                                 int32_t *label, int32_t *cid, int32_t *nn) {
-    if (dim < 1 || dim > 3) return fail(RTDBS_EINVAL, "dim must be 1, 2 or 3 (got %d)", dim);
-    if (!(eps > 0.0f) || !(eps < INFINITY)) return fail(RTDBS_EINVAL, "eps must be finite and > 0");
-    if (min_pts < 1) return fail(RTDBS_EINVAL, "min_pts must be >= 1 (got %d)", min_pts);
-    if (n < 1 || n > (int64_t)INT_MAX - kBfTile) return fail(RTDBS_EINVAL, "n out of range (got %lld)", (long long)n);
+    int rc = check_bf_args(dim, eps, min_pts, n);
+    if (rc) return rc;
     if (!points || !label || !cid) return fail(RTDBS_EINVAL, "NULL argument");
     const size_t N = (size_t)n;
-    std::vector<float> soa(3 * N, 0.0f);
-    for (size_t i = 0; i < N; i++)
-        for (int d = 0; d < dim; d++) soa[d * N + i] = points[i * dim + d];
-    int rc = RTDBS_OK;
     char *mem = nullptr;
     BfArgs a{};
-    const unsigned blocks = (unsigned)((N + kBfTile - 1) / kBfTile);
-    BFCHK(hipMalloc((void **)&mem, 3 * N * 4 + 5 * N * 4));
-    a.x = (const float *)mem;
-    a.y = a.x + N;
-    a.z = a.y + N;
-    a.nn = (int32_t *)(mem + 3 * N * 4);
+    BFCHK(hipMalloc((void **)&mem, N * (size_t)dim * 4 + 4 * N * 4));
+    a.pts = (const float *)mem;
+    a.nn = (int32_t *)(mem + N * (size_t)dim * 4);
     a.parent = a.nn + N;
     a.label = a.parent + N;
     a.cid = a.label + N;
@@ -160,15 +183,8 @@ extern "C" int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const f
     a.dim = dim;
     a.min_pts = min_pts;
     a.e2 = eps * eps;  // fl32(eps*eps), as the reference's radius2 (dbscan.cpp:41)
-    BFCHK(hipMemcpy(mem, soa.data(), 3 * N * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_bf_count, dim3(blocks), dim3(kBfTile), 0, 0, a);
-    BFCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_bf_union, dim3(blocks), dim3(kBfTile), 0, 0, a);
-    BFCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_bf_flatten, dim3(blocks), dim3(kBfTile), 0, 0, a);
-    BFCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_bf_border, dim3(blocks), dim3(kBfTile), 0, 0, a);
-    BFCHK(hipGetLastError());
+    BFCHK(hipMemcpy(mem, points, N * (size_t)dim * 4, hipMemcpyHostToDevice));
+    BFCHK(launch_bf(a, 0));
     BFCHK(hipMemcpy(label, a.label, N * 4, hipMemcpyDeviceToHost));
     BFCHK(hipMemcpy(cid, a.cid, N * 4, hipMemcpyDeviceToHost));
     if (nn) BFCHK(hipMemcpy(nn, a.nn, N * 4, hipMemcpyDeviceToHost));

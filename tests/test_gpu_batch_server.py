@@ -275,3 +275,44 @@ def test_server_evict_readmit_and_cancel(engine):
     l, c = srv.wait(srv.submit(sid, _stride(sp, d, 3)), 30000)
     assert np.array_equal(l, alone[3][0]) and np.array_equal(c, alone[3][1])
     srv.close()
+
+
+def test_server_swaps_under_a_binding_budget(engine):
+    """A budget that holds about half the streams: with swapping on, every
+    stream still opens, requests of streams not on the device swap them in
+    (least recently used idle streams out), device bytes never exceed the
+    budget, and every result equals the stream's plain result."""
+    R = engine
+    slides = 4
+    specs = _specs()
+    datas = [_data(sp, slides) for sp in specs]
+    plain = []
+    for sp, d in zip(specs, datas):
+        s = _stream(R, sp, d)
+        plain.append([tuple(x.copy() for x in s.slide(_stride(sp, d, t))) for t in range(slides)])
+        s.close()
+    sizes = [R.Stream.device_bytes(sp["dim"], sp["W"], sp["S"], sp["eps"], sp["mp"], cell_div=sp["k"])
+             for sp in specs]
+    budget = max(sizes) + sum(sorted(sizes)[:2])  # the largest stream fits, never all of them
+    assert budget < sum(sizes)
+    srv = R.Server(hbm_budget_bytes=budget, max_batch=4, lanes=2, swap=True)
+    sids = [srv.open_stream(sp["dim"], sp["W"], sp["S"], sp["eps"], sp["mp"], d[:sp["W"]], cell_div=sp["k"])
+            for sp, d in zip(specs, datas)]
+    assert srv.stats()["resident"] < len(specs)
+    for t in range(slides):
+        order = np.random.default_rng(t).permutation(len(specs))
+        tickets = [(int(i), srv.submit(sids[i], _stride(specs[i], datas[i], t))) for i in order]
+        for i, tk in tickets:
+            l, c = srv.wait(tk, 60000)
+            assert np.array_equal(l, plain[i][t][0]) and np.array_equal(c, plain[i][t][1]), (i, t)
+        st = srv.stats()
+        assert st["bytes_in_use"] <= budget
+    st = srv.stats()
+    assert st["swap_ins"] > 0 and st["swap_outs"] > 0 and st["failed"] == 0
+    with pytest.raises(R.RtdbsError) as e:  # larger than the whole budget
+        srv.open_stream(3, 200000, 20000, 0.5, 10, np.zeros((200000, 3), np.float32))
+    assert e.value.code == R.EBUDGET
+    for sid in sids:
+        srv.close_stream(sid)
+    assert srv.stats()["bytes_in_use"] == 0
+    srv.close()

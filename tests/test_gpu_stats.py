@@ -5,7 +5,8 @@
 - dense_points: points of interior cells holding >= MinPts points;
 - border_tests: every non-core point tests its whole stencil (no early exit),
   so the total is exactly the sum of the non-core points' stencil sizes;
-- count_tests: non-core queries test their whole stencil, core queries stop
+- count_tests: non-core queries test their whole stencil (those whose
+  stencil holds fewer than MinPts points may test nothing), core queries stop
   at MinPts or later, no query tests beyond its stencil;
 - and counting must not change any label.
 """
@@ -32,7 +33,9 @@ def expected_counts(windows, labels, eps, mp, info):
         e["dense_points"] += int(dense.sum())
         e["queries"] += int((~dense).sum())
         e["border_tests"] += int(sten[~core].sum())
-        e["count_lo"] += int(sten[~core].sum()) + mp * int((core & ~dense).sum())
+        # a non-core query whose stencil holds fewer than MinPts points may be
+        # settled without a test (nothing can make it core)
+        e["count_lo"] += int(sten[~core & (sten >= mp)].sum()) + mp * int((core & ~dense).sum())
         e["count_hi"] += int(sten[~dense].sum())
     return e
 

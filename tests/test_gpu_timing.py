@@ -8,7 +8,7 @@ from rtdbstream_amd import workloads as WL
 
 pytestmark = pytest.mark.gpu
 
-PHASES = ("sort_expire", "rank_compact", "merge_final", "cells", "rows", "count", "union", "flatten", "border")
+PHASES = ("sort_expire", "rank_compact", "merge_final", "cells", "rows", "count", "union", "border")
 
 
 def _stream(engine, W=4000, S=400, seed=5):
