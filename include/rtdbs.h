@@ -276,6 +276,18 @@ void rtdbs_dataset_free(rtdbs_dataset *ds);
 int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const float *points, int64_t n, int32_t *label,
                      int32_t *cid, int32_t *nn);
 
+/* rtdbs_bruteforce without any allocation: no reference counterpart. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Inputs and outputs
+ * are DEVICE buffers: points_dev (n * dim floats, point-major), label_dev,
+ * cid_dev and the optional nn_dev (n ints each). The union-find parents (and
+ * the neighbour counts when nn_dev is NULL) live in a fixed per-GPU scratch
+ * array of the library, so n is limited to 2^20 points (RTDBS_EINVAL beyond);
+ * every call on one GPU shares that array, so calls that overlap on one GPU
+ * race. Synchronous (null stream); same outputs as rtdbs_bruteforce. */
+int rtdbs_bruteforce_noalloc(int32_t dim, float eps, int32_t min_pts, const float *points_dev, int64_t n,
+                             int32_t *label_dev, int32_t *cid_dev, int32_t *nn_dev);
+
 /* Device-to-device copy by a plain kernel (bytes a multiple of 16): a kernel
  * of known traffic for scaling profiler byte counters. Synchronous. */
 int rtdbs_device_copy(void *dst_dev, const void *src_dev, size_t bytes);

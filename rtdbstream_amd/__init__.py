@@ -143,6 +143,8 @@ SIGNATURES = [
     ("rtdbs_dataset_free", None, [ctypes.POINTER(Dataset)]),
     ("rtdbs_bruteforce", ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P, ctypes.c_int64, _P, _P,
                                         _P]),
+    ("rtdbs_bruteforce_noalloc", ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _P, ctypes.c_int64,
+                                                _P, _P, _P]),
     ("rtdbs_device_synchronize", ctypes.c_int, []),
     ("rtdbs_device_count", ctypes.c_int, [_I32P]),
     ("rtdbs_batch_set_stats", ctypes.c_int, [_P, ctypes.c_int]),

@@ -18,6 +18,7 @@
 
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "rtdbs.h"
@@ -26,9 +27,16 @@
 
 namespace rtdbs {
 
+// Scratch of rtdbs_bruteforce_noalloc: union-find parents [0, kNoAllocMax)
+// and, when the caller passes no nn array, neighbour counts [kNoAllocMax,
+// 2 * kNoAllocMax). One copy per GPU, held until the process ends (8 MiB).
+constexpr int64_t kNoAllocMax = int64_t(1) << 20;
+static __device__ __align__(256) int32_t scratch[2 * kNoAllocMax];
+
 namespace {
 
 constexpr int kBfTile = 256;  // candidates staged per LDS tile = threads per block
+constexpr int kMaxDevices = 64;
 
 struct BfArgs {
     const float *pts;  // n points, dim floats each, point-major
@@ -160,6 +168,28 @@ int check_bf_args(int32_t dim, float eps, int32_t min_pts, int64_t n) {
         }                                                                            \
     } while (0)
 
+// Device address of this GPU's scratch array, looked up once per device with
+// the device current. Host code uses only this address, never the array's
+// name (which is a host-side stand-in, not a device address).
+int scratch_ptr(int32_t **out) {
+    static std::mutex mu;
+    static int32_t *ptr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(RTDBS_EHIP, "hipGetDevice failed: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(RTDBS_EHIP, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(RTDBS_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        ptr[dev] = (int32_t *)p;
+    }
+    *out = ptr[dev];
+    return RTDBS_OK;
+}
+
 }  // namespace
 }  // namespace rtdbs
 
@@ -191,4 +221,31 @@ extern "C" int rtdbs_bruteforce(int32_t dim, float eps, int32_t min_pts, const f
 done:
     if (mem) (void)hipFree(mem);
     return rc;
+}
+
+extern "C" int rtdbs_bruteforce_noalloc(int32_t dim, float eps, int32_t min_pts, const float *points_dev, int64_t n,
+                                        int32_t *label_dev, int32_t *cid_dev, int32_t *nn_dev) {
+    int rc = check_bf_args(dim, eps, min_pts, n);
+    if (rc) return rc;
+    if (n > kNoAllocMax)
+        return fail(RTDBS_EINVAL, "n = %lld exceeds the %lld points the allocation-free validator's scratch holds",
+                    (long long)n, (long long)kNoAllocMax);
+    if (!points_dev || !label_dev || !cid_dev) return fail(RTDBS_EINVAL, "NULL argument");
+    int32_t *scr = nullptr;
+    rc = scratch_ptr(&scr);
+    if (rc) return rc;
+    BfArgs a{};
+    a.pts = points_dev;
+    a.n = (int)n;
+    a.dim = dim;
+    a.min_pts = min_pts;
+    a.e2 = eps * eps;  // fl32(eps*eps), as the reference's radius2 (dbscan.cpp:41)
+    a.parent = scr;  // every entry < n is written by the count kernel before it is read
+    a.nn = nn_dev ? nn_dev : scr + kNoAllocMax;
+    a.label = label_dev;
+    a.cid = cid_dev;
+    hipError_t e = launch_bf(a, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(0);
+    if (e != hipSuccess) return fail(RTDBS_EHIP, "validator launch failed: %s", hipGetErrorString(e));
+    return RTDBS_OK;
 }
