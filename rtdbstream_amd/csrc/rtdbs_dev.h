@@ -18,8 +18,7 @@ constexpr int kChunk = 64;         // positions per wave in the neighbourhood ke
 constexpr int kCap = 512;          // candidates staged in LDS per wave
 constexpr int kMaxK = 3;           // largest stencil reach
 constexpr int kMaxRows = 49;       // (2k+1)^(dim-1) for k <= 3, dim <= 3
-constexpr int kKeyBits = 40;       // cell key bits; slot bits = 64 - 40 at most 24
-constexpr int kMaxSlotBits = 24;
+constexpr int kMaxSlotBits = 24;  // slot bits of the packed key; the cell key takes the rest (<= 52)
 constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
 // Per-stream work counters (optional; rtdbs_stats in the C ABI)

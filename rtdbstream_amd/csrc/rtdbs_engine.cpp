@@ -282,45 +282,79 @@ namespace rtdbs {
 
 // Grid geometry: side = eps/k * (1 + 2^-16). A pair accepted by the FP32 test
 // has true distance below eps*(1 + 2^-21), so on every axis the exact cell
-// indices differ by at most k; the double evaluation of the index adds at most
-// n*2^-52 <= 2^-22 cells for the per-axis caps below.
+// indices differ by at most k; the double evaluation of an index adds at most
+// 2^-22 cells for grids of up to 2^30 cells per axis.
+//
+// The packed cell key (i0 * n1 + i1) * n2 + i2 has no per-axis cap: each axis
+// gets the cells its extent needs, as long as their product fits the key bits
+// the window leaves (64 - slot bits, at most 52 so that a key is exact in a
+// double). Without given bounds the grid spans the first window with margins
+// for drift: twice its extent on each side (at least eps), or less where the
+// key bits run out. Only beyond that are the axes shrunk, proportionally, and
+// points outside clamped into the border cells, which keeps the search exact
+// (never dense or safe) but slower.
 static void set_geometry(rtdbs_stream *s, const float *pts) {
     const rtdbs_params &p = s->p;
     const int d = p.dim;
-    const int64_t cap = d == 3 ? (int64_t(1) << 13) : d == 2 ? (int64_t(1) << 20) : (int64_t(1) << 30);
     s->side = (double)p.eps / (double)s->k * (1.0 + 1.0 / 65536.0);
     const double inv = 1.0 / s->side;
+    const int key_bits = std::min(52, 64 - s->slot_bits);
+    const double key_cells = std::ldexp(1.0, key_bits);
+    const double axis_cap = std::ldexp(1.0, 30);
     bool given = true;
     for (int a = 0; a < d; a++)
         if (!(p.hi[a] > p.lo[a])) given = false;
-    for (int a = 0; a < 3; a++) {
-        s->lo[a] = 0.0;
-        s->n[a] = 1;
-    }
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     for (int a = 0; a < d; a++) {
-        double lo, hi;
         if (given) {
-            lo = p.lo[a];
-            hi = p.hi[a];
+            mn[a] = p.lo[a];
+            mx[a] = p.hi[a];
         } else {
-            float mn = INFINITY, mx = -INFINITY;
+            float lo = INFINITY, hi = -INFINITY;
             for (int i = 0; i < p.window; i++) {
                 const float v = pts[(size_t)i * d + a];
-                if (v < mn) mn = v;
-                if (v > mx) mx = v;
+                if (v < lo) lo = v;
+                if (v > hi) hi = v;
             }
-            if (!(mx >= mn)) mn = mx = 0.0f;  // all NaN
-            const double ext = (double)mx - (double)mn;
-            // room for the stream to drift; points outside are clamped into
-            // the border cells, which keeps the search exact
-            lo = (double)mn - 0.5 * ext - p.eps;
-            hi = (double)mx + 0.5 * ext + p.eps;
-            lo = (double)(float)lo;
+            if (!(hi >= lo)) lo = hi = 0.0f;  // all NaN
+            mn[a] = lo;
+            mx[a] = hi;
         }
-        s->lo[a] = lo;
-        double cells = std::floor((hi - lo) * inv) + 1.0;
-        if (!(cells >= 1.0)) cells = 1.0;
-        s->n[a] = cells > (double)cap ? cap : (int64_t)cells;
+    }
+    // cells per axis for bounds [mn - m*ext - pad, mx + m*ext + pad]
+    auto cells_for = [&](double m, double *lo_out, double *n_out) {
+        double prod = 1.0;
+        for (int a = 0; a < d; a++) {
+            const double ext = mx[a] - mn[a];
+            const double pad = given ? 0.0 : (double)p.eps;
+            double lo = mn[a] - m * ext - pad;
+            const double hi = mx[a] + m * ext + pad;
+            if (!given) lo = (double)(float)lo;
+            double cells = std::floor((hi - lo) * inv) + 1.0;
+            if (!(cells >= 1.0)) cells = 1.0;
+            cells = std::min(cells, axis_cap);
+            lo_out[a] = lo;
+            n_out[a] = cells;
+            prod *= cells;
+        }
+        return prod;
+    };
+    double lo[3] = {0, 0, 0}, n[3] = {1, 1, 1};
+    double m = given ? 0.0 : 2.0;
+    double prod = cells_for(m, lo, n);
+    while (!given && prod > key_cells && m > 0.25) {
+        m *= 0.5;
+        prod = cells_for(m, lo, n);
+    }
+    if (prod > key_cells) {
+        // still too many cells: shrink every axis by the same factor (the
+        // outermost points are then clamped into the border cells)
+        const double f = std::pow(key_cells / prod, 1.0 / d) * (1.0 - 1e-9);
+        for (int a = 0; a < d; a++) n[a] = std::max(1.0, std::floor(n[a] * f));
+    }
+    for (int a = 0; a < 3; a++) {
+        s->lo[a] = a < d ? lo[a] : 0.0;
+        s->n[a] = a < d ? (int64_t)n[a] : 1;
     }
     StreamDev &D = s->dev;
     D.inv_side = inv;

@@ -208,9 +208,10 @@ __device__ __forceinline__ uint64_t cell_key_g(const KeyGrid &G, float x, float 
     return (uint64_t)((i0 * G.n1 + i1) * G.n2 + i2);
 }
 
-// Quotient and remainder of a cell key (< 2^40) by a grid extent (< 2^31),
-// via a double-precision estimate and an exact integer fix-up (64-bit integer
-// division is a long software sequence on the GPU).
+// Quotient and remainder of a cell key (< 2^52, so exact in a double) by a
+// grid extent (<= 2^30), via a double-precision estimate, off by at most one,
+// and an exact integer fix-up (64-bit integer division is a long software
+// sequence on the GPU).
 __device__ __forceinline__ void divmod_key(uint64_t x, int64_t d, int64_t *q, int64_t *r) {
     int64_t qq = (int64_t)((double)x / (double)d);
     int64_t rr = (int64_t)x - qq * d;
