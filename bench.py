@@ -199,7 +199,7 @@ def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
     graph replay per slide), each device-resident and transfer-inclusive."""
     extra = 3
     total = warmup + 2 * steps + extra
-    specs = WL.multi_stream_specs(n_streams=n_streams, slides=total + 1)
+    specs = WL.multi_stream_specs(n_streams=n_streams, slides=total + 2)
     datas = [WL.stream_points(sp) for sp in specs]
     bufs = []
     for sp, d in zip(specs, datas):
@@ -311,6 +311,23 @@ def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
                    [x[4].addr() for x in bufs])
     bt.sync()
     phase_ms = {k: round(v[0] / max(1, v[1]), 4) for k, v in bt.timing().items() if v[1]}
+    # the merge's work in one more slide, counted (the counting kernels are not
+    # the timed ones): unite calls, touching element pairs, distance tests
+    bt.set_timing(0)
+    bt.set_stats(True)
+    bt.slide_async([in_addr(i, t_ph + 1) for i in range(len(streams))], [x[3].addr() for x in bufs],
+                   [x[4].addr() for x in bufs])
+    bt.sync()
+    tot = {"unites": 0, "merge_pairs": 0, "merge_tests": 0, "count_tests": 0, "count_queries": 0}
+    for i in range(len(streams)):
+        st = bt.stats(i)
+        for k in tot:
+            tot[k] += st[k]
+    bt.set_stats(False)
+    union_s = phase_ms.get("union", 0.0) / 1e3
+    merge = dict(tot, union_ms=phase_ms.get("union"),
+                 unites_per_s=round(tot["unites"] / union_s, 1) if union_s else None,
+                 pairs_per_s=round(tot["merge_pairs"] / union_s, 1) if union_s else None)
     b.close()
     bt.close()
     for s in streams:
@@ -332,6 +349,7 @@ def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
         "batched_step_p50_ms": pct(lat_b, 50), "batched_step_p99_ms": pct(lat_b, 99),
         "labels_equal_plain": bool(same_dev),
         "phase_ms": phase_ms,
+        "merge_work": merge,
         "transfer_inclusive": {
             "plain_points_per_s": round(pts / t_plain_x, 1), "plain_slides_per_s": round(nsl / t_plain_x, 1),
             "batched_points_per_s": round(pts / t_batch_x, 1), "batched_slides_per_s": round(nsl / t_batch_x, 1),
