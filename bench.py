@@ -76,13 +76,15 @@ def tune(R, dim, W, S, eps, mp, data, enabled):
     """Grid side and kernel form from the project's autotuner (trial slides
     on the stream's first W + 9S points); the defaults when disabled."""
     if not enabled:
-        return {"cell_div": 0, "lanes_per_query": 0, "neighbor_mode": 0, "from_cache": False, "trials": 0}
+        return {"cell_div": 0, "lanes_per_query": 0, "neighbor_mode": 0, "cellpar_per_cu": 0, "rows_per_cu": 0,
+                "from_cache": False, "trials": 0}
     return R.autotune(dim, W, S, eps, mp, data[:W + 9 * S], slides=8)
 
 
 def tuned_stream(R, sp, t, data):
     s = R.Stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], cell_div=t["cell_div"],
-                 lanes_per_query=t["lanes_per_query"], neighbor_mode=t["neighbor_mode"])
+                 lanes_per_query=t["lanes_per_query"], neighbor_mode=t["neighbor_mode"],
+                 cellpar_per_cu=t["cellpar_per_cu"], rows_per_cu=t["rows_per_cu"])
     s.init(data[:sp["window"]])
     return s
 
@@ -428,7 +430,8 @@ def plain_leg(R, WL, name, steps, warmup, autotune=True, roof_sample=4):
         "transfer_inclusive": {"points_per_s": round(steps * S / elx, 1), "ms_per_slide": round(elx / steps * 1e3, 4),
                                "p50_ms": pct(latx, 50), "p99_ms": pct(latx, 99)},
         "phase_ms": phase_ms,
-        "autotune": {k: tu[k] for k in ("cell_div", "lanes_per_query", "neighbor_mode", "from_cache", "trials")},
+        "autotune": {k: tu[k] for k in ("cell_div", "lanes_per_query", "neighbor_mode", "cellpar_per_cu",
+                                        "rows_per_cu", "from_cache", "trials")},
         "tune_s": round(tune_s, 2),
         "cells_last": info["cells"],
         "roofline": roofline_obj(B, ms / max(1, n), "k_count", "k_count_%s" % name),

@@ -63,7 +63,11 @@ typedef struct rtdbs_params {
     int32_t lanes_per_query; /* threads per query of the point-parallel kernels: 1, 2, 4; 0 = auto */
     int32_t neighbor_mode;   /* count/border kernels of a single-stream launch: 1 cell-parallel,
                                 2 point-parallel; 0 = by launch size */
-    int32_t reserved[5];
+    int32_t cellpar_per_cu;  /* workgroups per CU of the persistent cell-parallel kernels (count,
+                                merge, outputs) in a single-stream launch, 1..1024; 0 = by launch size */
+    int32_t rows_per_cu;     /* workgroups per CU of the stencil-rows kernel in a single-stream
+                                launch, 1..1024; 0 = by launch size */
+    int32_t reserved[3];
 } rtdbs_params;
 
 typedef struct rtdbs_stream rtdbs_stream;
@@ -140,10 +144,12 @@ void rtdbs_batch_destroy(rtdbs_batch *b);
  * dbscan.cpp:34). Times trial slides of the given stream sample for every
  * stencil reach k (grid side Eps/k), then for each form of the neighbourhood
  * kernels (cell-parallel; point-parallel with 1, 2 or 4 lanes per query) at
- * the best reach, and returns the fastest. `points` holds W + slides*S points (slides >= 1). Results are
- * cached by (dim, W, S, MinPts, density) in the file named by the
- * RTDBS_TUNE_CACHE environment variable (default
- * $HOME/.cache/rtdbs_autotune.txt); a cache hit runs no trials. */
+ * the best reach, then for the workgroups per CU of the cell-parallel kernels
+ * and then of the stencil-rows kernel, and returns the fastest setting.
+ * `points` holds W + (slides+1)*S points (slides >= 1: one warm-up slide, then
+ * `slides` timed ones per trial). Results are cached by (dim, W, S, MinPts,
+ * density) in the file named by the RTDBS_TUNE_CACHE environment variable
+ * (default $HOME/.cache/rtdbs_autotune.txt); a cache hit runs no trials. */
 typedef struct rtdbs_tune_result {
     int32_t cell_div;
     int32_t lanes_per_query;
@@ -152,6 +158,8 @@ typedef struct rtdbs_tune_result {
     int32_t trials;
     char key[96];
     int32_t neighbor_mode; /* 1 cell-parallel, 2 point-parallel (with lanes_per_query) */
+    int32_t cellpar_per_cu;/* workgroups per CU of the cell-parallel kernels */
+    int32_t rows_per_cu;   /* workgroups per CU of the stencil-rows kernel */
     int32_t pad;
 } rtdbs_tune_result;
 

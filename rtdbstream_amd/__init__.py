@@ -44,7 +44,7 @@ class Params(ctypes.Structure):
         ("eps", ctypes.c_float), ("min_pts", ctypes.c_int32),
         ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
         ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("neighbor_mode", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("cellpar_per_cu", ctypes.c_int32), ("rows_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -52,7 +52,8 @@ class TuneResult(ctypes.Structure):
     _fields_ = [
         ("cell_div", ctypes.c_int32), ("lanes_per_query", ctypes.c_int32), ("ms_per_slide", ctypes.c_double),
         ("from_cache", ctypes.c_int32), ("trials", ctypes.c_int32), ("key", ctypes.c_char * 96),
-        ("neighbor_mode", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("neighbor_mode", ctypes.c_int32), ("cellpar_per_cu", ctypes.c_int32), ("rows_per_cu", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -199,7 +200,7 @@ def device_count():
 
 
 def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0,
-                neighbor_mode=0):
+                neighbor_mode=0, cellpar_per_cu=0, rows_per_cu=0):
     p = Params()
     p.dim, p.window, p.stride, p.eps, p.min_pts = int(dim), int(window), int(stride), float(eps), int(min_pts)
     for a in range(3):
@@ -208,17 +209,21 @@ def make_params(dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0,
     p.cell_div = int(cell_div)
     p.lanes_per_query = int(lanes_per_query)
     p.neighbor_mode = int(neighbor_mode)
+    p.cellpar_per_cu = int(cellpar_per_cu)
+    p.rows_per_cu = int(rows_per_cu)
     return p
 
 
 def autotune(dim, window, stride, eps, min_pts, points, slides=3, lo=None, hi=None):
-    """Stencil reach and lanes-per-query for this stream, from timed trial
-    slides on `points` (W + (slides+1)*S points) or the tuning cache."""
+    """Stencil reach, kernel form and persistent grid sizes for this stream,
+    from timed trial slides on `points` (W + (slides+1)*S points) or the
+    tuning cache."""
     p = make_params(dim, window, stride, eps, min_pts, lo, hi)
     pts = np.ascontiguousarray(points, dtype=np.float32)
     r = TuneResult()
     _check(lib().rtdbs_autotune(ctypes.byref(p), _ptr(pts), pts.size // int(dim), int(slides), ctypes.byref(r)))
     return {"cell_div": r.cell_div, "lanes_per_query": r.lanes_per_query, "neighbor_mode": r.neighbor_mode,
+            "cellpar_per_cu": r.cellpar_per_cu, "rows_per_cu": r.rows_per_cu,
             "ms_per_slide": r.ms_per_slide, "from_cache": bool(r.from_cache), "trials": r.trials,
             "key": r.key.decode()}
 
@@ -336,17 +341,17 @@ class Stream:
     """One sliding window (the reference's ScanState + slide loop body)."""
 
     def __init__(self, dim, window, stride, eps, min_pts, lo=None, hi=None, cell_div=0, lanes_per_query=0,
-                 neighbor_mode=0):
+                 neighbor_mode=0, cellpar_per_cu=0, rows_per_cu=0):
         self.params = make_params(dim, window, stride, eps, min_pts, lo, hi, cell_div, lanes_per_query,
-                                  neighbor_mode)
+                                  neighbor_mode, cellpar_per_cu, rows_per_cu)
         self.dim, self.W, self.S = int(dim), int(window), int(stride)
         h = ctypes.c_void_p()
         _check(lib().rtdbs_stream_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h.value
 
     @staticmethod
-    def device_bytes(dim, window, stride, eps, min_pts, cell_div=0):
-        p = make_params(dim, window, stride, eps, min_pts, cell_div=cell_div)
+    def device_bytes(dim, window, stride, eps, min_pts, cell_div=0, **kw):
+        p = make_params(dim, window, stride, eps, min_pts, cell_div=cell_div, **kw)
         n = ctypes.c_size_t()
         _check(lib().rtdbs_stream_device_bytes(ctypes.byref(p), ctypes.byref(n)))
         return n.value

@@ -21,6 +21,7 @@ constexpr int kMaxRows = 49;       // (2k+1)^(dim-1) for k <= 3, dim <= 3
 constexpr int kMaxSlotBits = 24;  // slot bits of the packed key; the cell key takes the rest (<= 52)
 constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
+constexpr int kMaxPerCu = 1024;  // largest workgroups-per-CU setting of a persistent grid
 // Per-stream work counters (optional; rtdbs_stats in the C ABI)
 constexpr int kNumStats = 8;
 enum Stat {

@@ -106,6 +106,10 @@ int validate_params(const rtdbs_params *p) {
         return fail(RTDBS_EINVAL, "lanes_per_query must be 0 (auto), 1, 2 or 4 (got %d)", p->lanes_per_query);
     if (p->neighbor_mode < 0 || p->neighbor_mode > 2)
         return fail(RTDBS_EINVAL, "neighbor_mode must be 0 (auto), 1 or 2 (got %d)", p->neighbor_mode);
+    if (p->cellpar_per_cu < 0 || p->cellpar_per_cu > kMaxPerCu)
+        return fail(RTDBS_EINVAL, "cellpar_per_cu must be 0 (auto) or 1..%d (got %d)", kMaxPerCu, p->cellpar_per_cu);
+    if (p->rows_per_cu < 0 || p->rows_per_cu > kMaxPerCu)
+        return fail(RTDBS_EINVAL, "rows_per_cu must be 0 (auto) or 1..%d (got %d)", kMaxPerCu, p->rows_per_cu);
     for (int a = 0; a < p->dim; a++)
         if (!std::isfinite(p->lo[a]) || !std::isfinite(p->hi[a]))
             return fail(RTDBS_EINVAL, "grid bounds must be finite");
@@ -624,9 +628,12 @@ int refresh_batch(rtdbs_batch *b) {
     const long long cap = (long long)device_cus() * 8;
     long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
-    // rows: a persistent grid of RTDBS_ROWS_PER_CU blocks per CU (measured
-    // best: 8 for a latency-bound launch, 32 for a large batch)
-    const int rcap = env_int("RTDBS_ROWS_PER_CU", cells_total <= kCellParMaxPoints ? 8 : 32, 1, 1 << 20);
+    // a single stream's own tuned grid sizes (0: by launch size)
+    const rtdbs_params *p1 = n == 1 ? &b->streams[0]->p : nullptr;
+    // rows: a persistent grid of blocks per CU (measured best: 8 for a
+    // latency-bound launch, 32 for a large batch); RTDBS_ROWS_PER_CU overrides
+    const int r_def = p1 && p1->rows_per_cu > 0 ? p1->rows_per_cu : (cells_total <= kCellParMaxPoints ? 8 : 32);
+    const int rcap = env_int("RTDBS_ROWS_PER_CU", r_def, 1, kMaxPerCu);
     b->shape.rows_blocks = (int)std::max(1ll, std::min((long long)device_cus() * rcap, pb));
     // Count and border: cell-parallel (coalesced candidate loads, more
     // instructions per query) while a launch is small and latency-bound,
@@ -647,9 +654,11 @@ int refresh_batch(rtdbs_batch *b) {
     // cell-parallel grid: 16 blocks per CU while the launch is small; 32 for
     // large batches, where it carries the merge alone (64 streams: 455 ->
     // ~400 us). RTDBS_CELLPAR_PER_CU overrides.
-    const int cp_def = cells_total <= kCellParMaxPoints ? 16 : 32;
+    const int cp_def =
+        p1 && p1->cellpar_per_cu > 0 ? p1->cellpar_per_cu : (cells_total <= kCellParMaxPoints ? 16 : 32);
     b->shape.cellpar_blocks = (int)std::max(
-        1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", cp_def, 1, 1024), (cells_total + 3) / 4));
+        1ll, std::min((long long)device_cus() * env_int("RTDBS_CELLPAR_PER_CU", cp_def, 1, kMaxPerCu),
+                      (cells_total + 3) / 4));
     b->shape.cell_blocks = (int)std::max(
         1ll, std::min((long long)device_cus() * waves_per_cu(), (cells_total + kUGCells - 1) / kUGCells));
     // lanes per query: enough threads to cover the device several times over,

@@ -1,7 +1,8 @@
-// Autotuner: picks the grid side (stencil reach k, side = Eps/k) and the
-// form of the neighbourhood kernels (cell-parallel, or point-parallel with 1,
-// 2 or 4 lanes per query) by timing trial slides on a sample of the stream,
-// and remembers the choice by a workload signature.
+// Autotuner: picks the grid side (stencil reach k, side = Eps/k), the form
+// of the neighbourhood kernels (cell-parallel, or point-parallel with 1, 2 or
+// 4 lanes per query) and the grid sizes of the persistent kernels by timing
+// trial slides on a sample of the stream, and remembers the choice by a
+// workload signature.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,12 +65,12 @@ std::string make_key(const rtdbs_params &p, const float *pts) {
 }
 
 struct Choice {
-    int k = 2, lpq = 0, mode = 0;
+    int k = 2, lpq = 0, mode = 0, cp = 0, rows = 0;
     double ms = 0.0;
 };
 
-// Cache lines: "key k lanes_per_query neighbor_mode ms"; the last entry of a
-// key wins.
+// Cache lines: "key k lanes_per_query neighbor_mode ms cellpar_per_cu
+// rows_per_cu"; the last complete entry of a key wins.
 bool cache_lookup(const std::string &key, Choice *c) {
     FILE *f = std::fopen(cache_path().c_str(), "r");
     if (!f) return false;
@@ -77,7 +78,8 @@ bool cache_lookup(const std::string &key, Choice *c) {
     bool hit = false;
     while (std::fgets(line, sizeof(line), f)) {
         Choice x;
-        if (std::sscanf(line, "%127s %d %d %d %lf", kb, &x.k, &x.lpq, &x.mode, &x.ms) == 5 && key == kb) {
+        if (std::sscanf(line, "%127s %d %d %d %lf %d %d", kb, &x.k, &x.lpq, &x.mode, &x.ms, &x.cp, &x.rows) == 7 &&
+            key == kb) {
             *c = x;
             hit = true;
         }
@@ -96,7 +98,7 @@ void cache_store(const std::string &key, const Choice &c) {
     }
     FILE *f = std::fopen(path.c_str(), "a");
     if (!f) return;  // read-only home: tuning still returns its result
-    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), c.k, c.lpq, c.mode, c.ms);
+    std::fprintf(f, "%s %d %d %d %.6f %d %d\n", key.c_str(), c.k, c.lpq, c.mode, c.ms, c.cp, c.rows);
     std::fclose(f);
 }
 
@@ -157,6 +159,8 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
         out->cell_div = c.k;
         out->lanes_per_query = c.lpq;
         out->neighbor_mode = c.mode;
+        out->cellpar_per_cu = c.cp;
+        out->rows_per_cu = c.rows;
         out->ms_per_slide = c.ms;
         out->from_cache = 1;
         return RTDBS_OK;
@@ -197,9 +201,45 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
             best.lpq = fm[1];
         }
     }
+    // 3. workgroups per CU of the persistent cell-parallel kernels (the merge
+    // always runs on that grid), then of the stencil-rows kernel, each with the
+    // settings chosen so far
+    auto pick = [&](const int *vals, int nv, int Choice::*field) -> int {
+        double bm = 1e300;
+        int bv = vals[0];
+        for (int i = 0; i < nv; i++) {
+            Choice t = best;
+            t.*field = vals[i];
+            rtdbs_params p = p0;
+            p.cell_div = t.k;
+            p.neighbor_mode = t.mode;
+            p.lanes_per_query = t.lpq;
+            p.cellpar_per_cu = t.cp;
+            p.rows_per_cu = t.rows;
+            double m = 0.0;
+            const int r = trial(p, points, slides, &m);
+            if (r) return r;
+            out->trials++;
+            if (m < bm) {
+                bm = m;
+                bv = vals[i];
+            }
+        }
+        best.*field = bv;
+        best.ms = bm;
+        return RTDBS_OK;
+    };
+    static const int kCellparPerCu[] = {8, 12, 16, 24, 32, 48};
+    static const int kRowsPerCu[] = {4, 8, 16, 32};
+    rc = pick(kCellparPerCu, 6, &Choice::cp);
+    if (rc) return rc;
+    rc = pick(kRowsPerCu, 4, &Choice::rows);
+    if (rc) return rc;
     out->cell_div = best.k;
     out->lanes_per_query = best.lpq;
     out->neighbor_mode = best.mode;
+    out->cellpar_per_cu = best.cp;
+    out->rows_per_cu = best.rows;
     out->ms_per_slide = best.ms;
     cache_store(key, best);
     return RTDBS_OK;

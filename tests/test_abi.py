@@ -50,6 +50,10 @@ def test_version_and_error_string():
     (dict(cell_div=4), "cell_div"),
     (dict(lanes_per_query=3), "lanes_per_query"),
     (dict(neighbor_mode=3), "neighbor_mode"),
+    (dict(cellpar_per_cu=-1), "cellpar_per_cu"),
+    (dict(cellpar_per_cu=1025), "cellpar_per_cu"),
+    (dict(rows_per_cu=-2), "rows_per_cu"),
+    (dict(rows_per_cu=4096), "rows_per_cu"),
     (dict(window=(1 << 24) + 10, stride=10), "window"),
 ])
 def test_invalid_params_rejected(kw, msg):
@@ -66,6 +70,8 @@ def test_device_bytes_is_host_only():
     n2 = R.Stream.device_bytes(2, 100_000, 10_000, 0.5, 10)
     assert n3 > n2 > 100_000 * 40
     assert R.Stream.device_bytes(3, 100_000, 10_000, 0.5, 10, cell_div=1) < n3
+    # grid sizes shape launches only, not the stream's memory
+    assert R.Stream.device_bytes(3, 100_000, 10_000, 0.5, 10, cellpar_per_cu=48, rows_per_cu=4) == n3
 
 
 def test_null_arguments_rejected():

@@ -166,17 +166,35 @@ def test_autotune_choice_is_exact(engine, oracle, tmp_path, monkeypatch):
     r = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
     assert r["cell_div"] in (1, 2, 3) and r["neighbor_mode"] in (1, 2)
     assert r["lanes_per_query"] in ((0,) if r["neighbor_mode"] == 1 else (1, 2, 4))
-    assert not r["from_cache"] and r["trials"] == 7 and r["ms_per_slide"] > 0
+    assert r["cellpar_per_cu"] in (8, 12, 16, 24, 32, 48) and r["rows_per_cu"] in (4, 8, 16, 32)
+    # 3 reaches, 4 kernel forms, 6 cell-parallel and 4 rows grid sizes
+    assert not r["from_cache"] and r["trials"] == 17 and r["ms_per_slide"] > 0
     r2 = engine.autotune(3, W, S, 0.3, 6, data, slides=slides)
     assert r2["from_cache"] and r2["trials"] == 0
-    assert (r2["cell_div"], r2["neighbor_mode"], r2["lanes_per_query"]) == \
-        (r["cell_div"], r["neighbor_mode"], r["lanes_per_query"])
+    keys = ("cell_div", "neighbor_mode", "lanes_per_query", "cellpar_per_cu", "rows_per_cu")
+    assert [r2[k] for k in keys] == [r[k] for k in keys]
     ref_l, ref_c = oracle.stream(data, W, S, 0.3, 6, 5, method="grid")
     s = engine.Stream(3, W, S, 0.3, 6, cell_div=r["cell_div"], lanes_per_query=r["lanes_per_query"],
-                      neighbor_mode=r["neighbor_mode"])
+                      neighbor_mode=r["neighbor_mode"], cellpar_per_cu=r["cellpar_per_cu"],
+                      rows_per_cu=r["rows_per_cu"])
     s.init(data[:W])
     for t in range(5):
         l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
+@pytest.mark.parametrize("cp,rows", [(1, 1), (8, 4), (48, 32), (1024, 1024)])
+def test_grid_sizes_agree(engine, oracle, cp, rows):
+    # workgroups per CU of the persistent kernels change how cells are dealt
+    # out, never the result: from one workgroup per CU (every wave walks many
+    # cells) to more workgroups than cells
+    data = _mixed(4000 + 4 * 400, 3, 43)
+    ref_l, ref_c = oracle.stream(data, 4000, 400, 0.3, 6, 4, method="grid")
+    s = engine.Stream(3, 4000, 400, 0.3, 6, cellpar_per_cu=cp, rows_per_cu=rows)
+    s.init(data[:4000])
+    for t in range(4):
+        l, c = s.slide(data[4000 + t * 400: 4000 + (t + 1) * 400])
         assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
     s.close()
 
