@@ -137,13 +137,17 @@ static Layout make_layout(const rtdbs_params &p, int R) {
     const size_t ntile = (W + kWTile - 1) / kWTile;
     L.off_mut = take(sizeof(StreamMut));
     L.off_skey = take(W * 8);
-    L.off_sx = take(W * 4);
-    L.off_sy = d > 1 ? take(W * 4) : 0;
-    L.off_sz = d > 2 ? take(W * 4) : 0;
+    // window coordinates (both buffers of the ping-pong) with kCandPad
+    // readable floats past the end: the point-parallel kernels load candidates
+    // in groups that may run past a stencil row's end
+    const size_t wc = (W + kCandPad) * 4;
+    L.off_sx = take(wc);
+    L.off_sy = d > 1 ? take(wc) : 0;
+    L.off_sz = d > 2 ? take(wc) : 0;
     L.off_akey = take(W * 8);
-    L.off_ax = take(W * 4);
-    L.off_ay = d > 1 ? take(W * 4) : 0;
-    L.off_az = d > 2 ? take(W * 4) : 0;
+    L.off_ax = take(wc);
+    L.off_ay = d > 1 ? take(wc) : 0;
+    L.off_az = d > 2 ? take(wc) : 0;
     L.off_bkey0 = take(S * 8);
     L.off_bx0 = take(S * 4);
     L.off_by0 = d > 1 ? take(S * 4) : 0;

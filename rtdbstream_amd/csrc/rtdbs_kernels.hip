@@ -1187,6 +1187,19 @@ __device__ __forceinline__ CellView cell_view(const StreamDev &S) {
 
 __device__ __forceinline__ int vslot(const CellView &V, int pos) { return (int)((uint32_t)V.skey[pos] & V.smask); }
 
+// Element at byte offset `off` of a wave-uniform array: the load takes the
+// array's scalar base and a 32-bit lane offset, with no 64-bit address
+// arithmetic per lane (window positions are below 2^24, so every offset into
+// a per-position array fits).
+template <class T>
+__device__ __forceinline__ T ldo(gptr<const T> base, uint32_t off) {
+    return *(gptr<const T>)((gptr<const char>)base + off);
+}
+template <class T>
+__device__ __forceinline__ T ldo(gptr<T> base, uint32_t off) {
+    return *(gptr<const T>)((gptr<const char>)base + off);
+}
+
 constexpr int kRowGroup = 5;  // stencil rows fetched together
 constexpr int kUnroll = 4;    // candidates fetched together
 
@@ -1941,29 +1954,70 @@ __global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu
 }
 
 // Range count, point-parallel form (large batches, where the device is
-// saturated and per-query work matters more than coalescing): dense cells are core without counting
-// (all pairs within eps); otherwise the LPQ lanes of a query count its
-// neighbours (itself included) over its cell's stencil rows, each stopping
-// once its own count reaches min_pts. Queries of one cell are adjacent and
-// read the same candidates.
-template <int LPQ>
-__device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
-    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
-    const int s = uni(ql.x.s);
-    const CellView V = cell_view(a.s[s]);
+// saturated and per-query work matters more than coalescing): dense cells are
+// core without counting (all pairs within eps); otherwise the LPQ lanes of a
+// query count its neighbours (itself included) over its cell's stencil rows,
+// each stopping once its own count reaches min_pts. Queries of one cell are
+// adjacent and read the same candidates. D is the stream's dimension, so the
+// distance test and the loads carry no per-candidate dimension selects.
+//
+// One lane's share of a query's stencil: rows first, first + step, ...,
+// kRowGroup rows per round trip, kUnroll candidates per round trip. A group
+// of candidates is loaded whole, without clamping to its row's end: the loads
+// past the end read the next positions or the arrays' padding (kCandPad) and
+// are masked out of the count. The count is checked against min_pts once per
+// group.
+template <int D>
+__device__ __forceinline__ int count_walk(const CellView &V, int c, float qx, float qy, float qz, int first,
+                                          int step, int &tests) {
+    const int R = V.R, minpts = V.min_pts;
+    const float e2 = V.eps2;
+    int cnt = 0;
+    for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
+        int rx[kRowGroup], ry[kRowGroup];
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            const uint64_t rw = (r0 + i * step < R) ? V.rows[(size_t)c * R + r0 + i * step] : 0ull;
+            rx[i] = (int)(uint32_t)rw;
+            ry[i] = (int)(uint32_t)(rw >> 32);
+        }
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            for (int j = rx[i]; j < ry[i]; j += kUnroll) {
+                const uint32_t o = (uint32_t)j << 2;
+                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    cx[u] = ldo(V.sx, o + 4u * u);
+                    cy[u] = D > 1 ? ldo(V.sy, o + 4u * u) : 0.f;
+                    cz[u] = D > 2 ? ldo(V.sz, o + 4u * u) : 0.f;
+                }
+                const int nv = ry[i] - j;
+                tests += min(nv, kUnroll);
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) cnt += (u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2);
+                if (cnt >= minpts) return cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+template <int LPQ, int D>
+__device__ __forceinline__ void count_vblock(const BatchArgs &a, const QueryLane<LPQ> &ql, int s,
+                                             const CellView &V) {
     const int p = ql.p;
     const bool valid = p < ql.x.p1;
-    const int dim = V.dim, minpts = V.min_pts;
-    const float e2 = V.eps2;
+    const int minpts = V.min_pts;
     int cnt = 0;
     bool dense = false;
     int c = 0;
-    // the query's cell and coordinates and its slot, one round trip
-    const int pp = valid ? p : ql.x.p0;
-    const int cv = V.pcell[pp];
-    const float qx = V.sx[pp];
-    const float qy = dim > 1 ? V.sy[pp] : 0.f;
-    const float qz = dim > 2 ? V.sz[pp] : 0.f;
+    // the query's cell and coordinates, one round trip
+    const uint32_t po = (uint32_t)(valid ? p : ql.x.p0) << 2;
+    const int cv = ldo(V.pcell, po);
+    const float qx = ldo(V.sx, po);
+    const float qy = D > 1 ? ldo(V.sy, po) : 0.f;
+    const float qz = D > 2 ? ldo(V.sz, po) : 0.f;
     arrived(qx);
     arrived(qy);
     arrived(qz);
@@ -1972,16 +2026,7 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
         dense = V.cflag[c] & kCellDense;
     }
     int tests = 0;
-    if (valid && !dense) {
-        walk_candidates(
-            V, c,
-            [&](int, float cx, float cy, float cz) {
-                tests++;
-                cnt += sqd(dim, qx, qy, qz, cx, cy, cz) < e2;
-                return cnt >= minpts;
-            },
-            ql.sub, LPQ);
-    }
+    if (valid && !dense) cnt = count_walk<D>(V, c, qx, qy, qz, ql.sub, LPQ, tests);
     if (a.stats) {
         const unsigned long long t = (unsigned long long)wave_sum(tests);
         const int nd = __popcll(__ballot(valid && dense && ql.sub == 0));
@@ -1999,30 +2044,76 @@ __device__ __forceinline__ void count_vblock(const BatchArgs &a, int vb) {
 
 template <int LPQ>
 __global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
-    count_vblock<LPQ>(a, blockIdx.x);
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a, blockIdx.x);
+    const int s = uni(ql.x.s);
+    const CellView V = cell_view(a.s[s]);
+    if (V.dim == 3) count_vblock<LPQ, 3>(a, ql, s, V);
+    else if (V.dim == 2) count_vblock<LPQ, 2>(a, ql, s, V);
+    else count_vblock<LPQ, 1>(a, ql, s, V);
 }
 
 // Border / noise, point-parallel form (large batches): LPQ lanes per
 // non-core point, which takes the minimum root among the cores within eps.
 // The kernel also writes the cores' outputs (the root of each core's
 // union-find element, found here: no separate flatten pass) and advances each
-// stream's ring cursor, as the cell-parallel output kernel does.
-template <int LPQ>
-__device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
-    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
-    const int s = uni(ql.x.s);
-    const CellView V = cell_view(a.s[s]);
+// stream's ring cursor, as the cell-parallel output kernel does. D is the
+// stream's dimension.
+//
+// One lane's share of a non-core point's stencil, walked as count_walk walks
+// it (groups loaded past a row's end, masked), without an early exit: the
+// minimum root of the cores within eps.
+template <int D>
+__device__ __forceinline__ int border_walk(const CellView &V, const int32_t *parent, int c, float qx, float qy,
+                                           float qz, int first, int step, int &tests) {
+    const int R = V.R;
+    const float e2 = V.eps2;
+    int best = INT_MAX;
+    for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
+        int rx[kRowGroup], ry[kRowGroup];
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            const uint64_t rw = (r0 + i * step < R) ? V.rows[(size_t)c * R + r0 + i * step] : 0ull;
+            rx[i] = (int)(uint32_t)rw;
+            ry[i] = (int)(uint32_t)(rw >> 32);
+        }
+#pragma unroll
+        for (int i = 0; i < kRowGroup; i++) {
+            for (int j = rx[i]; j < ry[i]; j += kUnroll) {
+                const uint32_t o = (uint32_t)j << 2;
+                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    cx[u] = ldo(V.sx, o + 4u * u);
+                    cy[u] = D > 1 ? ldo(V.sy, o + 4u * u) : 0.f;
+                    cz[u] = D > 2 ? ldo(V.sz, o + 4u * u) : 0.f;
+                }
+                const int nv = ry[i] - j;
+                tests += min(nv, kUnroll);
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++)
+                    if ((u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2) &&
+                        (ldo(V.flags, (uint32_t)(j + u)) & kFlagCore))
+                        best = min(best, uf_root(parent, ldo(V.rep, o + 4u * u)));
+            }
+        }
+    }
+    return best;
+}
+
+template <int LPQ, int D>
+__device__ __forceinline__ void border_vblock(const BatchArgs &a, const QueryLane<LPQ> &ql, int vb, int s,
+                                              const CellView &V) {
     const int p = ql.p;
     // the query's flags, cell, coordinates and slot, one round trip
     const int pp = p < ql.x.p1 ? p : ql.x.p0;
-    const int qf = V.flags[pp];
-    const int c = V.pcell[pp];
-    const int dim = V.dim;
-    const float qx = V.sx[pp];
-    const float qy = dim > 1 ? V.sy[pp] : 0.f;
-    const float qz = dim > 2 ? V.sz[pp] : 0.f;
-    const uint64_t qk = V.skey[pp];
-    const int qr = V.rep[pp];
+    const uint32_t po = (uint32_t)pp << 2;
+    const int qf = ldo(V.flags, (uint32_t)pp);
+    const int c = ldo(V.pcell, po);
+    const float qx = ldo(V.sx, po);
+    const float qy = D > 1 ? ldo(V.sy, po) : 0.f;
+    const float qz = D > 2 ? ldo(V.sz, po) : 0.f;
+    const uint64_t qk = ldo(V.skey, po << 1);
+    const int qr = ldo(V.rep, po);
     arrived(c);
     arrived(qx);
     arrived(qy);
@@ -2043,20 +2134,9 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
         gl(io.cid)[slot] = uf_root(parent, qr);
     }
     const bool mine = p < ql.x.p1 && !(qf & kFlagCore);
-    const float e2 = V.eps2;
     int best = INT_MAX;
     int tests = 0;
-    if (mine) {
-        walk_candidates(
-            V, c,
-            [&](int j, float cx, float cy, float cz) {
-                tests++;
-                if (sqd(dim, qx, qy, qz, cx, cy, cz) < e2 && (V.flags[j] & kFlagCore))
-                    best = min(best, uf_root(parent, V.rep[j]));
-                return false;
-            },
-            ql.sub, LPQ);
-    }
+    if (mine) best = border_walk<D>(V, parent, c, qx, qy, qz, ql.sub, LPQ, tests);
     if (a.stats) {
         const unsigned long long t = (unsigned long long)wave_sum(tests);
         if (lane_id() == 0 && t) atomicAdd(a.stats + (size_t)s * kNumStats + kStatBorderTests, t);
@@ -2071,7 +2151,13 @@ __device__ __forceinline__ void border_vblock(const BatchArgs &a, int vb) {
 
 template <int LPQ>
 __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
-    border_vblock<LPQ>(a, blockIdx.x);
+    const int vb = blockIdx.x;
+    const QueryLane<LPQ> ql = query_lane<LPQ>(a, vb);
+    const int s = uni(ql.x.s);
+    const CellView V = cell_view(a.s[s]);
+    if (V.dim == 3) border_vblock<LPQ, 3>(a, ql, vb, s, V);
+    else if (V.dim == 2) border_vblock<LPQ, 2>(a, ql, vb, s, V);
+    else border_vblock<LPQ, 1>(a, ql, vb, s, V);
 }
 
 // First (minimum-slot) core of every cell - its union-find representative -
