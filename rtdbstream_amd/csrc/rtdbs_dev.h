@@ -104,7 +104,12 @@ struct BatchArgs {
     const int32_t *pre_chunk;  // n+1 prefix of ceil(W/kChunk)
     unsigned long long *stats; // n * kNumStats per-stream counters, null when off
     int32_t *cell_pre;         // n+1 prefix of cells per stream (written by k_rows)
+    int32_t xcd_groups;        // kXcd* bits: kernels whose XCDs each take one contiguous range of work
 };
+
+// BatchArgs::xcd_groups bits
+constexpr int32_t kXcdUnion = 1;         // k_union
+constexpr int32_t kXcdBorderPoints = 2;  // k_border_points
 
 // Kernel launchers (rtdbs_kernels.hip). All enqueue on `st` and return the
 // hipError_t of the launch.

@@ -688,6 +688,10 @@ int refresh_batch(rtdbs_batch *b) {
     a.pre_itile = b->d_pre + 3 * (n + 1);
     a.stats = b->stats_on ? b->d_stats : nullptr;
     a.cell_pre = b->d_cell_pre;
+    // XCD-grouped work order: in large batches for the merge and the
+    // point-parallel output kernel, where it pays (RTDBS_XCD_GROUPS sets the
+    // kXcd* bits)
+    a.xcd_groups = env_int("RTDBS_XCD_GROUPS", cells_total > kCellParMaxPoints ? kXcdUnion | kXcdBorderPoints : 0, 0, 3);
     if (b->gexec) {
         (void)hipGraphExecDestroy(b->gexec);
         b->gexec = nullptr;

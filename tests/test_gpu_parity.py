@@ -199,6 +199,24 @@ def test_grid_sizes_agree(engine, oracle, cp, rows):
     s.close()
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_xcd_grouped_work_order_agrees(engine, oracle, mode, monkeypatch):
+    # the merge and the point-parallel output kernel with each XCD's
+    # workgroups on one contiguous range of cells or positions; W = 3000 gives
+    # grids that are not multiples of the 8 XCDs (750 cell-parallel blocks,
+    # 47 chunks), so the regrouping must cover every block exactly once
+    monkeypatch.setenv("RTDBS_XCD_GROUPS", "3")
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    data = _mixed(3000 + 4 * 300, 3, 44)
+    ref_l, ref_c = oracle.stream(data, 3000, 300, 0.3, 5, 4, method="grid")
+    s = engine.Stream(3, 3000, 300, 0.3, 5)
+    s.init(data[:3000])
+    for t in range(4):
+        l, c = s.slide(data[3000 + t * 300: 3000 + (t + 1) * 300])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
 @pytest.mark.parametrize("lpq", [1, 2, 4])
 def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
     # lanes per query shape the point-parallel count and border kernels
