@@ -952,6 +952,17 @@ __device__ __forceinline__ uint8_t cell_class(const RowsView &V, int64_t i0, int
     return cnt >= V.min_pts ? (kCellSafe | kCellDense) : kCellSafe;
 }
 
+// Storage order of a cell's stencil rows: nearest first, group by group. An
+// offset d in [-k, k] ranks as 0, -1, +1, -2, +2, ... (|d| first, the
+// negative side first). In 3-D row (d1, d2) - offsets on the first two axes -
+// is stored at rank(d1) * (2k+1) + rank(d2): the rows of one first-axis
+// offset (one rows-kernel item) stay adjacent, and the groups come nearest
+// first. In 2-D row d1 is stored at rank(d1). The centre row is row 0. The
+// count kernels walk rows in storage order, so a query's count meets the
+// rows holding most of its neighbours first and reaches min_pts sooner; the
+// other consumers need no particular order.
+__device__ __forceinline__ int offset_rank(int d) { return d == 0 ? 0 : 2 * (d < 0 ? -d : d) - (d < 0 ? 1 : 0); }
+
 // Rows per item: in 3-D one item holds the 2k+1 rows of one offset o0 on the
 // first axis (o1 = -k..k on the second), whose target columns are adjacent,
 // so one bracket serves all of them; in 2-D and 1-D an item is one row.
@@ -972,7 +983,7 @@ __device__ __forceinline__ void rows_tile(const RowsView &V, int t, int g, uint6
     int64_t q, i0, i1, i2;
     divmod_key(ck, V.n2, &q, &i2);
     divmod_key((uint64_t)q, V.n1, &i0, &i1);
-    const int k = V.k, w = 2 * k + 1, center = (V.R - 1) / 2;
+    const int k = V.k, w = 2 * k + 1;
     const int nsub = V.dim == 3 ? w : 1;
     // offsets of the item's rows (uniform): o0 fixed, o1 over [o1lo, o1lo + nsub)
     int o0 = 0, o1lo = 0;
@@ -1058,9 +1069,8 @@ __device__ __forceinline__ void rows_tile(const RowsView &V, int t, int g, uint6
                 if (fg < eg) row = (uint64_t)(uint32_t)V.cstart[fg] | ((uint64_t)(uint32_t)V.cstart[eg] << 32);
             }
         }
-        // stencil row number, centre row first
-        const int rr = V.dim == 3 ? g * w + j : g;
-        const int r = rr == center ? 0 : (rr == 0 ? center : rr);
+        // stencil row number: nearest groups and rows first, the centre row 0
+        const int r = V.dim == 3 ? offset_rank(o0) * w + offset_rank(o1lo + j) : (V.dim == 2 ? offset_rank(o0) : 0);
         if (cv) {
             V.rows[(size_t)c * V.R + r] = row;
             if (r == 0) {
