@@ -1813,19 +1813,23 @@ constexpr int kCellLevelMin = 192;
 // elements, and safe cells whose first core is out of reach) have their
 // points scanned.
 template <class ST>
-__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const CellHead &h, ST &st, EdgeBuf &eb) {
+__device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const CellHead &h, ST &st, EdgeBuf &eb,
+                           uint64_t rw) {
     const int lane = lane_id();
     const int dim = V.dim;
     const float e2 = V.eps2;
-    // the header comes screened (cell_head); the rows load with the first
-    // queries, then each candidate chunk is a round of its own
-    const int A = h.c;
+    // the header comes screened (cell_head) and the stencil rows (rw, lane r:
+    // row r, every lane loaded) came in with the previous cell; the first
+    // queries and the first candidate chunk load in one round, then each
+    // further chunk is a round of its own
     const bool safeA = h.cf & kCellSafe, denseA = h.cf & kCellDense;
     const int cs = h.cs, nA = h.ce - h.cs;
-    const uint64_t rw = row_word(V, A);
-    const uint64_t rk = V.skey[h.fc];  // the representative's slot
-    RowWalk w0;
-    bool by_cells = false;
+    const RowWalk w0 = row_walk(lane < V.R ? rw : 0ull, safeA ? cs : cs + nA);
+    if (w0.rest == 0ull) return;
+    const bool by_cells = safeA && uni(wave_sum(w0.len)) >= kCellLevelMin;
+    // a safe cell's element: its representative's slot, which every position
+    // of the cell holds in rep (unsafe cells: each core its own slot)
+    const int repA = uni(V.rep[cs]);
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1833,40 +1837,35 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
         const float qx = V.sx[pq];
         const float qy = dim > 1 ? V.sy[pq] : 0.f;
         const float qz = dim > 2 ? V.sz[pq] : 0.f;
-        const uint64_t qk = V.skey[pq];
+        const int qslot = V.rep[pq];  // read for the cores of unsafe cells: the point's own slot
+        // the point scan's first chunk, in the same round (not for the
+        // cell-level pass, which scans other walks)
+        RowWalk w1 = w0;
+        int end1 = 0;
+        const int pos1 = walk_chunk(w1, 0, &end1);
+        const int f1 = by_cells ? 0 : V.flags[pos1];
+        const float x1 = by_cells ? 0.f : V.sx[pos1];
+        const float y1 = by_cells || dim < 2 ? 0.f : V.sy[pos1];
+        const float z1 = by_cells || dim < 3 ? 0.f : V.sz[pos1];
+        const int r1 = by_cells ? 0 : V.rep[pos1];
         arrived(qx);
         arrived(qy);
         arrived(qz);
-        arrived(qk);
-        if (q0 == 0) {
-            arrived(rw);
-            w0 = row_walk(rw, safeA ? cs : cs + nA);
-            if (w0.rest == 0ull) return;
-            by_cells = safeA && uni(wave_sum(w0.len)) >= kCellLevelMin;
-        }
+        arrived(qslot);
+        arrived(x1);
+        arrived(y1);
+        arrived(z1);
+        arrived(r1);
         const bool qcore = lane < nq && (denseA || (qf & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
-        const int qslot = (int)((uint32_t)qk & V.smask);
-        const int repA = uni((int)((uint32_t)rk & V.smask));
-        // The cores qm of this block against the candidate positions of walk w.
-        auto scan = [&](RowWalk w) __attribute__((always_inline)) {
-            for (int jb = 0;; jb += 64) {
-                int end;
-                const int pos = walk_chunk(w, jb, &end);
-                if (jb >= end) break;
-                const int f = V.flags[pos];
-                const float cx = V.sx[pos];
-                const float cy = dim > 1 ? V.sy[pos] : 0.f;
-                const float cz = dim > 2 ? V.sz[pos] : 0.f;
-                const int rp = V.rep[pos];  // meaningful for cores only
-                arrived(cx);
-                arrived(cy);
-                arrived(cz);
-                arrived(rp);
+        // The cores qm of this block against candidate chunk [jb, jb + 64) of
+        // a walk whose candidates end at `end`.
+        auto test = [&](const int jb, const int end, const int pos, const int f, const float cx, const float cy,
+                        const float cz, const int rp) __attribute__((always_inline)) {
                 const bool bcore = jb + lane < end && (f & kFlagCore);
                 const uint64_t bm = __ballot(bcore);
-                if (bm == 0ull) continue;
+                if (bm == 0ull) return;
                 const int rep = bcore ? rp : 0;
                 const unsigned chunk = (unsigned)min(64, end - jb);
                 uint64_t m = qm;
@@ -1903,10 +1902,43 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
                         unite_once_per_rep(a, eb, s, hit, rep, lane_i(qslot, qi), st);
                     }
                 }
+        };
+        // the cores qm of this block against the candidate positions of walk w
+        auto scan = [&](RowWalk w) __attribute__((always_inline)) {
+            for (int jb = 0;; jb += 64) {
+                int end;
+                const int pos = walk_chunk(w, jb, &end);
+                if (jb >= end) break;
+                const int f = V.flags[pos];
+                const float cx = V.sx[pos];
+                const float cy = dim > 1 ? V.sy[pos] : 0.f;
+                const float cz = dim > 2 ? V.sz[pos] : 0.f;
+                const int rp = V.rep[pos];  // meaningful for cores only
+                arrived(cx);
+                arrived(cy);
+                arrived(cz);
+                arrived(rp);
+                test(jb, end, pos, f, cx, cy, cz, rp);
             }
         };
         if (!by_cells) {
-            scan(w0);
+            // the first chunk came with the queries
+            if (end1 > 0) test(0, end1, pos1, f1, x1, y1, z1, r1);
+            for (int jb = 64;; jb += 64) {
+                int end;
+                const int pos = walk_chunk(w1, jb, &end);
+                if (jb >= end) break;
+                const int f = V.flags[pos];
+                const float cx = V.sx[pos];
+                const float cy = dim > 1 ? V.sy[pos] : 0.f;
+                const float cz = dim > 2 ? V.sz[pos] : 0.f;
+                const int rp = V.rep[pos];
+                arrived(cx);
+                arrived(cy);
+                arrived(cz);
+                arrived(rp);
+                test(jb, end, pos, f, cx, cy, cz, rp);
+            }
             continue;
         }
         // candidate cells: row r (lane r) covers positions [rx, rx + len), all
@@ -1933,7 +1965,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
             const float bx = V.sx[fp];
             const float by = dim > 1 ? V.sy[fp] : 0.f;
             const float bz = dim > 2 ? V.sz[fp] : 0.f;
-            const uint64_t bk = V.skey[fp];
+            const int bk = V.rep[fp];  // a safe cell's element: its first core's slot
             arrived(bx);
             arrived(by);
             arrived(bz);
@@ -1948,7 +1980,7 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
                 hit |= __ballot(sqd(dim, lane_f(qx, qi), lane_f(qy, qi), lane_f(qz, qi), bx, by, bz) < e2) & cand;
             }
             // a safe cell's element is its first core's slot; one lane per cell
-            unite_once_per_rep(a, eb, s, (hit >> lane) & 1ull, (int)((uint32_t)bk & V.smask), repA, st);
+            unite_once_per_rep(a, eb, s, (hit >> lane) & 1ull, bk, repA, st);
             // the cells with cores not settled: scan their points
             RowWalk pw;
             pw.rx = bs;
@@ -1967,13 +1999,30 @@ __global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu
     ST st;
     EdgeBuf eb{edges[threadIdx.x >> 6], 0};
     const int w = uni(xcd_block(blockIdx.x, gridDim.x, a.xcd_groups & kXcdUnion) * kCellWaves + (threadIdx.x >> 6));
+    // stencil rows of a cell, lane r: row r (every lane loads, clamped to the
+    // last row; lanes >= R are cleared where the rows are used)
+    auto rows_of = [&](const CellHead &x) __attribute__((always_inline)) {
+        const StreamDev &S = a.s[x.s];
+        const int R = ldc(S.R);
+        return gl((const uint64_t *)ldc(S.rows))[(size_t)x.c * R + min(lane_id(), R - 1)];
+    };
     for (long long g0 = w; g0 < total; g0 += 64ll * nw) {
         const CellHead mine = cell_head(a, g0 + (long long)lane_id() * nw, total);
-        for (uint64_t m = __ballot(mine.fc != kNoCore); m; m &= m - 1ull) {
-            const int j = __ffsll((unsigned long long)m) - 1;
-            const CellHead h = lane_head(mine, j);
+        uint64_t m = __ballot(mine.fc != kNoCore);
+        if (m == 0ull) continue;
+        CellHead h = lane_head(mine, __ffsll((unsigned long long)m) - 1);
+        uint64_t rw = rows_of(h);
+        while (true) {
+            m &= m - 1ull;
+            // the next cell's rows are in flight while this cell merges (past
+            // the last cell the current one's reload: no branch around the load)
+            const CellHead hn = m ? lane_head(mine, __ffsll((unsigned long long)m) - 1) : h;
+            const uint64_t rwn = rows_of(hn);
             stats_stream(a, st, h.s);
-            union_cell(a, h.s, cell_view(a.s[h.s]), h, st, eb);
+            union_cell(a, h.s, cell_view(a.s[h.s]), h, st, eb, rw);
+            if (m == 0ull) break;
+            h = hn;
+            rw = rwn;
         }
     }
     edge_flush(a, eb);
