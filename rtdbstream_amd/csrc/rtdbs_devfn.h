@@ -77,6 +77,12 @@ __device__ __forceinline__ int xor_lane(int v, int m) {
     }
 }
 
+// v of lane (this lane - 1), lane 0 gets 0: one DPP move across the whole
+// wave (wave_shr:1), no LDS round trip, unlike __shfl_up (ds_bpermute).
+__device__ __forceinline__ int prev_lane(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, xor_lane(v, o));

@@ -1872,8 +1872,8 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
                 if (safeA) {
                     // lanes holding one element form a run (a safe cell's points are
                     // adjacent); one touching lane settles its whole run
-                    const int prev_rep = __shfl_up(rep, 1);  // every lane takes part in the shuffles
-                    const int prev_core = __shfl_up((int)bcore, 1);
+                    const int prev_rep = prev_lane(rep);  // every lane takes part in the moves
+                    const int prev_core = prev_lane((int)bcore);
                     const uint64_t starts = __ballot(lane == 0 || prev_rep != rep || !prev_core);
                     const uint64_t after = starts & ~((2ull << lane) - 1ull);  // run starts above this lane
                     const uint64_t upto = after ? (after & (~after + 1ull)) - 1ull : ~0ull;
