@@ -1,8 +1,8 @@
 """Pins the CPU oracle (oracle/rtdbs_oracle.c) before it is trusted.
 
-The reference ships no tests, fixtures or golden vectors for this path
-(SURVEY: "no test directory, no fixtures"), and its code needs CUDA/OptiX, so
-it cannot be built here. The oracle is pinned instead by:
+The reference ships no tests, fixtures or golden vectors for this path (its
+repository has no test directory), and its code needs CUDA/OptiX, so it
+cannot be built here. The oracle is pinned instead by:
   * scikit-learn's DBSCAN on the exact FP32 squared distances (independent
     implementation; see tests/refcheck.py for why it is exact),
   * closed forms on lattices with exact ties at d2 == eps2,
