@@ -14,7 +14,9 @@ Also on the same JSON line:
   transfer_inclusive - the same two legs timed the reference's way
                   (dbscan.cpp:2289 stride H2D through :2344-2345 label/cid
                   D2H, into pinned host buffers, :54-55): strides from pinned
-                  host memory, labels and ids back in pinned host memory
+                  host memory, labels and ids back in pinned host memory;
+                  and the batch once more through pipelined slides, whose
+                  transfers overlap the next slide's kernels
   plain         - single-stream legs: walks3d (W 100,000) and skewed3d
                   (W 1,000,000), device-resident and transfer-inclusive
   roofline      - the range-count kernel of the 64-stream launch: algorithmic
@@ -296,6 +298,39 @@ def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
     t_batch_x = time.perf_counter() - a0
     same_x = all(np.array_equal(l, pl) and np.array_equal(c, pc)
                  for l, c, (pl, pc) in zip(tx.labels, tx.cids, plain_last))
+    # --- transfer-inclusive, pipelined (rtdbs_batch_slide_pipelined): fresh
+    # streams brought to the same windows by device slides, then the same
+    # pinned strides, outputs alternating between two pinned sets; one
+    # slide's transfers overlap the next slide's kernels
+    ps = [tuned_stream(R, sp, tu, d) for sp, d, tu in zip(specs, datas, tuned)]
+    bp = R.Batch(ps, use_graph=True)
+    for t in range(warmup + steps):
+        bp.slide_async([in_addr(i, t) for i in range(len(ps))], [x[3].addr() for x in bufs],
+                       [x[4].addr() for x in bufs])
+    bp.sync()
+    hout2 = R.HostBuffer(tx.hout.nbytes)
+    outsets = [(tx.labels, tx.cids), ([], [])]
+    o = 0
+    for sp in specs:
+        W = sp["window"]
+        outsets[1][0].append(hout2.array(np.int32, W, offset=o))
+        outsets[1][1].append(hout2.array(np.int32, W, offset=o + W * 4))
+        o += 2 * W * 4
+    pipe_warm = 2  # the first calls set up the double-buffered areas and capture their graphs
+    for k in range(steps):
+        if k == pipe_warm:
+            a0 = time.perf_counter()
+        bp.slide_pipelined(tx.ins[k], *outsets[k % 2])
+    bp.drain()
+    t_pipe_x = time.perf_counter() - a0
+    n_pipe = steps - pipe_warm
+    last = outsets[(steps - 1) % 2]
+    same_p = all(np.array_equal(l, pl) and np.array_equal(c, pc)
+                 for l, c, (pl, pc) in zip(last[0], last[1], plain_last))
+    bp.close()
+    for s in ps:
+        s.close()
+    hout2.free()
     tx.free()
     # --- range-count kernel of the batched launch: HIP events on the batch's
     # own stream, a few more slides without the graph
@@ -359,8 +394,13 @@ def batched_leg(R, WL, n_streams, steps, warmup, autotune=True):
             "plain_p50_ms": pct(lat_plain_x, 50), "plain_p99_ms": pct(lat_plain_x, 99),
             "batched_step_p50_ms": pct(lat_bx, 50), "batched_step_p99_ms": pct(lat_bx, 99),
             "labels_equal_plain": bool(same_x),
+            "pipelined_points_per_s": round(pts_per_slide * n_pipe / t_pipe_x, 1),
+            "pipelined_slides_per_s": round(len(specs) * n_pipe / t_pipe_x, 1),
+            "pipelined_speedup": round((t_plain_x / nsl) / (t_pipe_x / (len(specs) * n_pipe)), 3),
+            "pipelined_labels_equal_plain": bool(same_p),
             "host_buffers": "pinned (rtdbs_host_alloc); per slide one stride H2D and one label+cid D2H "
-                            "for the batch, three copies per plain slide"},
+                            "for the batch, three copies per plain slide; pipelined: the same copies on two "
+                            "copy streams, overlapping the next slide's kernels"},
         "_count": (cnt_ms / max(1, cnt_n), t_extra, specs, datas, infos),
     })
     return out
@@ -694,7 +734,8 @@ def main():
             "slides_per_s": bres["batched_slides_per_s"],
             "p50_ms": bres["batched_step_p50_ms"], "p99_ms": bres["batched_step_p99_ms"],
             "exact_labels": {"batched_vs_plain": bres["labels_equal_plain"],
-                             "transfer_inclusive_vs_plain": bres["transfer_inclusive"]["labels_equal_plain"]},
+                             "transfer_inclusive_vs_plain": bres["transfer_inclusive"]["labels_equal_plain"],
+                             "pipelined_vs_plain": bres["transfer_inclusive"]["pipelined_labels_equal_plain"]},
             "transfer_inclusive": dict(bres["transfer_inclusive"]),
             "roofline": roofline_obj(B, avg_ms, "k_count_points (64-stream launch)", "k_count_points_64streams"),
             "batched": bres,

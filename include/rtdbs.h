@@ -137,6 +137,21 @@ int rtdbs_batch_slide(rtdbs_batch *b, const float *const *stride_points, int32_t
 int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *stride_points_dev,
                             int32_t *const *labels_dev, int32_t *const *cids_dev);
 int rtdbs_batch_sync(rtdbs_batch *b);
+/* Pipelined host-buffer slide. No reference counterpart: the reference's loop
+ * copies the stride up, clusters and copies the labels down one after the
+ * other (dbscan.cpp:2289, :2344-2345). This call enqueues one slide of every
+ * member - the strides go up on one copy stream, the slide runs on the batch's
+ * stream, labels and ids come down on another copy stream, through two device
+ * areas used by alternate slides - and returns once the PREVIOUS pipelined
+ * slide's labels and ids are in the buffers given with that call. So one
+ * slide's transfers overlap the next slide's kernels. The buffers of this call
+ * must stay untouched until the next pipelined call, rtdbs_batch_drain or any
+ * other slide or sync of the batch returns; all of them must come from
+ * rtdbs_host_alloc (RTDBS_EINVAL otherwise). */
+int rtdbs_batch_slide_pipelined(rtdbs_batch *b, const float *const *stride_points, int32_t *const *labels,
+                                int32_t *const *cids);
+/* Waits until the last pipelined slide's labels and ids are in its buffers. */
+int rtdbs_batch_drain(rtdbs_batch *b);
 void rtdbs_batch_destroy(rtdbs_batch *b);
 
 /* ---- autotuner ------------------------------------------------------------ */

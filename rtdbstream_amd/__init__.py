@@ -119,6 +119,8 @@ SIGNATURES = [
     ("rtdbs_batch_slide", ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     ("rtdbs_batch_slide_async", ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     ("rtdbs_batch_sync", ctypes.c_int, [_P]),
+    ("rtdbs_batch_slide_pipelined", ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    ("rtdbs_batch_drain", ctypes.c_int, [_P]),
     ("rtdbs_batch_destroy", None, [_P]),
     ("rtdbs_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     ("rtdbs_server_open_stream", ctypes.c_int, [_P, ctypes.POINTER(Params), _P, _I32P]),
@@ -455,6 +457,29 @@ class Batch:
         C = (ctypes.c_void_p * n)(*[a.ctypes.data for a in cids])
         _check(lib().rtdbs_batch_slide(self.h, A, Lb, C))
         return labels, cids
+
+    def slide_pipelined(self, strides, labels, cids):
+        """Pipelined slide (rtdbs_batch_slide_pipelined): every array must
+        live in a HostBuffer (engine-pinned memory). Returns once the previous
+        pipelined call's labels and ids are in its arrays; this call's arrays
+        must stay untouched until the next pipelined call, drain() or any
+        other slide or sync of the batch."""
+        n = len(self.streams)
+        if len(strides) != n or len(labels) != n or len(cids) != n:
+            raise ValueError("one stride, label and cid array per stream")
+        for x, s in zip(strides, self.streams):
+            if x.dtype != np.float32 or x.size != s.S * s.dim or not x.flags.c_contiguous:
+                raise ValueError("stride arrays must be contiguous float32 of S*dim values")
+        labels = [_out_i32(x, s.W, "label") for x, s in zip(labels, self.streams)]
+        cids = [_out_i32(x, s.W, "cid") for x, s in zip(cids, self.streams)]
+        A = (ctypes.c_void_p * n)(*[a.ctypes.data for a in strides])
+        Lb = (ctypes.c_void_p * n)(*[a.ctypes.data for a in labels])
+        C = (ctypes.c_void_p * n)(*[a.ctypes.data for a in cids])
+        _check(lib().rtdbs_batch_slide_pipelined(self.h, A, Lb, C))
+
+    def drain(self):
+        """Waits for the last pipelined slide's outputs."""
+        _check(lib().rtdbs_batch_drain(self.h))
 
     def slide_async(self, stride_ptrs, label_ptrs, cid_ptrs):
         n = len(self.streams)

@@ -557,9 +557,31 @@ int batch_set_members(rtdbs_batch *b, rtdbs_stream *const *streams, int n) {
     return refresh_batch(b);
 }
 
+static void pipe_drop_graphs(rtdbs_batch *b) {
+    for (int p = 0; p < 2; p++) {
+        if (b->pipe.gexec[p]) (void)hipGraphExecDestroy(b->pipe.gexec[p]);
+        if (b->pipe.graph[p]) (void)hipGraphDestroy(b->pipe.graph[p]);
+        b->pipe.gexec[p] = nullptr;
+        b->pipe.graph[p] = nullptr;
+    }
+}
+
 void batch_free(rtdbs_batch *b) {
     if (!b) return;
+    (void)pipe_drain(b);
+    if (b->pipe.up) (void)hipStreamSynchronize(b->pipe.up);
+    if (b->pipe.down) (void)hipStreamSynchronize(b->pipe.down);
     if (b->hs) (void)hipStreamSynchronize(b->hs);
+    pipe_drop_graphs(b);
+    for (int p = 0; p < 2; p++) {
+        if (b->pipe.area[p]) (void)hipFree(b->pipe.area[p]);
+        for (hipEvent_t e : {b->pipe.in_done[p], b->pipe.kern_done[p], b->pipe.out_done[p]})
+            if (e) (void)hipEventDestroy(e);
+    }
+    if (b->pipe.d_io) (void)hipFree(b->pipe.d_io);
+    if (b->pipe.h_io) (void)hipHostFree(b->pipe.h_io);
+    if (b->pipe.up) (void)hipStreamDestroy(b->pipe.up);
+    if (b->pipe.down) (void)hipStreamDestroy(b->pipe.down);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipEvent_t e : b->ev_ring)
@@ -597,6 +619,8 @@ int refresh_batch(rtdbs_batch *b) {
     const int n = b->n;
     // the device may still read the tables (and the copy from their pinned
     // mirrors may be pending)
+    int rc0 = pipe_drain(b);
+    if (rc0) return rc0;
     if (b->hs) HIPCHK(hipStreamSynchronize(b->hs));
     StreamDev *desc = b->h_desc;
     int32_t *pre = b->h_pre;
@@ -700,6 +724,8 @@ int refresh_batch(rtdbs_batch *b) {
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
+    pipe_drop_graphs(b);
+    b->pipe.ready = false;  // the areas' layout and tables follow the members
     b->io_valid = false;
     return RTDBS_OK;
 }
@@ -777,13 +803,24 @@ int batch_collect_timing(rtdbs_batch *b) {
 }
 
 int batch_sync(rtdbs_batch *b) {
+    int rc = pipe_drain(b);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(b->hs));
+    return RTDBS_OK;
+}
+
+int pipe_drain(rtdbs_batch *b) {
+    if (!b->pipe.waiting) return RTDBS_OK;
+    b->pipe.waiting = false;
+    HIPCHK(hipEventSynchronize(b->pipe.out_done[(b->pipe.issued - 1) & 1]));
     return RTDBS_OK;
 }
 
 // One slide of all member streams with device I/O (table in d_io).
 int batch_launch(rtdbs_batch *b, const IO *io) {
-    int rc = batch_check_current(b);
+    int rc = pipe_drain(b);
+    if (rc) return rc;
+    rc = batch_check_current(b);
     if (rc) return rc;
     bool changed = !b->io_valid;
     for (int i = 0; i < b->n && !changed; i++)
@@ -904,7 +941,9 @@ static int ensure_xfer(rtdbs_batch *b) {
 }
 
 int batch_slide_host(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids) {
-    int rc = batch_check_current(b);
+    int rc = pipe_drain(b);
+    if (rc) return rc;
+    rc = batch_check_current(b);
     if (rc) return rc;
     rc = ensure_xfer(b);
     if (rc) return rc;
@@ -946,6 +985,133 @@ int batch_slide_host(rtdbs_batch *b, const float *const *in, int32_t *const *lab
     rc = batch_sync(b);
     if (rc) return rc;
     for (const Copy &c : post) std::memcpy(c.dst, c.src, c.bytes);
+    return RTDBS_OK;
+}
+
+// Areas, tables, streams and events of the pipelined slides, for the current
+// members (the caller has drained the pipe).
+static int pipe_setup(rtdbs_batch *b) {
+    auto &P = b->pipe;
+    if (P.ready) return RTDBS_OK;
+    int rc = ensure_xfer(b);  // the transfer area's layout (in_off, out_off)
+    if (rc) return rc;
+    const int n = b->n;
+    const size_t total = b->out_off[n - 1] + (size_t)b->streams[n - 1]->p.window * 8;
+    for (hipStream_t s : {P.up, P.down})
+        if (s) HIPCHK(hipStreamSynchronize(s));
+    if (total > P.area_bytes) {
+        for (int p = 0; p < 2; p++) {
+            if (P.area[p]) (void)hipFree(P.area[p]);
+            P.area[p] = nullptr;
+        }
+        P.area_bytes = 0;
+        const size_t want = std::max(total, total + total / 2);
+        for (int p = 0; p < 2; p++) {
+            hipError_t e = hipMalloc((void **)&P.area[p], want);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(RTDBS_ENOMEM, "pipelined transfer area of %zu bytes: %s", want, hipGetErrorString(e));
+            }
+        }
+        P.area_bytes = want;
+    }
+    if (!P.d_io) {
+        HIPCHK(hipMalloc((void **)&P.d_io, (size_t)2 * b->capacity * sizeof(IO)));
+        HIPCHK(hipHostMalloc((void **)&P.h_io, (size_t)2 * b->capacity * sizeof(IO), hipHostMallocDefault));
+    }
+    if (!P.up) {
+        HIPCHK(hipStreamCreateWithFlags(&P.up, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&P.down, hipStreamNonBlocking));
+        for (int p = 0; p < 2; p++)
+            for (hipEvent_t *e : {&P.in_done[p], &P.kern_done[p], &P.out_done[p]})
+                HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    for (int p = 0; p < 2; p++)
+        for (int i = 0; i < n; i++) {
+            char *area = P.area[p];
+            int32_t *label = (int32_t *)(area + b->out_off[i]);
+            P.h_io[p * b->capacity + i] =
+                IO{(const float *)(area + b->in_off[i]), label, label + b->streams[i]->p.window};
+        }
+    HIPCHK(hipMemcpyAsync(P.d_io, P.h_io, (size_t)2 * b->capacity * sizeof(IO), hipMemcpyHostToDevice, b->hs));
+    HIPCHK(hipStreamSynchronize(b->hs));
+    pipe_drop_graphs(b);
+    P.issued = 0;
+    P.ready = true;
+    return RTDBS_OK;
+}
+
+// One pipelined slide (rtdbs_batch_slide_pipelined). Area p = slide index & 1:
+//   up stream:      [wait: kernels of slide t-2 done]  strides -> area p      -> in_done[p]
+//   batch's stream: [wait: in_done[p], outputs of slide t-2 copied]  slide    -> kern_done[p]
+//   down stream:    [wait: kern_done[p]]  labels, ids of area p -> host       -> out_done[p]
+// then waits for out_done of slide t-1. So slide t's kernels run while slide
+// t-1's outputs go down and slide t+1's strides go up.
+int batch_slide_pipelined(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids) {
+    if (b->solo) return fail(RTDBS_EINVAL, "pipelined slides are for batches");
+    int rc = batch_check_current(b);  // may refresh (which drains the pipe)
+    if (rc) return rc;
+    for (int i = 0; i < b->n; i++) {
+        const rtdbs_stream *s = b->streams[i];
+        const size_t ni = (size_t)s->p.stride * s->p.dim * 4, no = (size_t)s->p.window * 4;
+        if (!host_pinned(in[i], ni) || !host_pinned(labels[i], no) || !host_pinned(cids[i], no))
+            return fail(RTDBS_EINVAL, "pipelined slides take rtdbs_host_alloc buffers only (stream %d)", i);
+    }
+    if (!b->pipe.ready) {
+        rc = pipe_drain(b);
+        if (rc) return rc;
+        rc = pipe_setup(b);
+        if (rc) return rc;
+    }
+    auto &P = b->pipe;
+    const int p = (int)(P.issued & 1);
+    char *area = P.area[p];
+    std::vector<Copy> up, down;
+    for (int i = 0; i < b->n; i++) {
+        const rtdbs_stream *s = b->streams[i];
+        add_copy(up, area + b->in_off[i], in[i], (size_t)s->p.stride * s->p.dim * 4);
+    }
+    for (int i = 0; i < b->n; i++) {
+        const size_t nb = (size_t)b->streams[i]->p.window * 4;
+        add_copy(down, labels[i], area + b->out_off[i], nb);
+        add_copy(down, cids[i], area + b->out_off[i] + nb, nb);
+    }
+    if (P.issued >= 2) HIPCHK(hipStreamWaitEvent(P.up, P.kern_done[p], 0));
+    for (const Copy &c : up) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, P.up));
+    HIPCHK(hipEventRecord(P.in_done[p], P.up));
+    HIPCHK(hipStreamWaitEvent(b->hs, P.in_done[p], 0));
+    if (P.issued >= 2) HIPCHK(hipStreamWaitEvent(b->hs, P.out_done[p], 0));
+    BatchArgs a = b->args;
+    a.use_io1 = 0;
+    a.io = P.d_io + (size_t)p * b->capacity;
+    if (b->use_graph) {
+        if (!P.gexec[p]) {
+            HIPCHK(hipStreamBeginCapture(b->hs, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launch_slide(a, b->shape, b->hs, nullptr, 0u);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(b->hs, &g);
+            if (le != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(RTDBS_EHIP, "graph capture launch failed: %s", hipGetErrorString(le));
+            }
+            if (ce != hipSuccess) return fail(RTDBS_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ce));
+            P.graph[p] = g;
+            HIPCHK(hipGraphInstantiate(&P.gexec[p], g, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(P.gexec[p], b->hs));
+    } else {
+        HIPCHK(launch_slide(a, b->shape, b->hs, nullptr, 0u));
+    }
+    HIPCHK(hipEventRecord(P.kern_done[p], b->hs));
+    HIPCHK(hipStreamWaitEvent(P.down, P.kern_done[p], 0));
+    for (const Copy &c : down) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, P.down));
+    HIPCHK(hipEventRecord(P.out_done[p], P.down));
+    for (auto *s : b->streams) s->slides++;
+    if (b->stats_on) b->stats_slides++;
+    const bool prev = P.waiting;
+    P.issued++;
+    P.waiting = true;
+    if (prev) HIPCHK(hipEventSynchronize(P.out_done[p ^ 1]));  // the previous slide's outputs
     return RTDBS_OK;
 }
 
@@ -993,6 +1159,19 @@ extern "C" int rtdbs_batch_slide_async(rtdbs_batch *b, const float *const *strid
     return batch_launch(b, io.data());
 }
 
+extern "C" int rtdbs_batch_slide_pipelined(rtdbs_batch *b, const float *const *stride_points, int32_t *const *labels,
+                                           int32_t *const *cids) {
+    if (!b || !stride_points || !labels || !cids) return fail(RTDBS_EINVAL, "NULL argument");
+    for (int i = 0; i < b->n; i++)
+        if (!stride_points[i] || !labels[i] || !cids[i]) return fail(RTDBS_EINVAL, "NULL buffer for stream %d", i);
+    return batch_slide_pipelined(b, stride_points, labels, cids);
+}
+
+extern "C" int rtdbs_batch_drain(rtdbs_batch *b) {
+    if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
+    return pipe_drain(b);
+}
+
 extern "C" int rtdbs_batch_sync(rtdbs_batch *b) {
     if (!b) return fail(RTDBS_EINVAL, "batch is NULL");
     return batch_sync(b);
@@ -1012,6 +1191,7 @@ extern "C" int rtdbs_batch_set_stats(rtdbs_batch *b, int enable) {
     b->stats_slides = 0;
     b->args.stats = b->stats_on ? b->d_stats : nullptr;
     // a captured graph holds the previous argument
+    pipe_drop_graphs(b);
     if (b->gexec) {
         (void)hipGraphExecDestroy(b->gexec);
         b->gexec = nullptr;

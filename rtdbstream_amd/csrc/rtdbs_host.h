@@ -65,6 +65,24 @@ struct rtdbs_batch {
     char *h_stage = nullptr;
     size_t h_stage_bytes = 0;
     std::vector<size_t> in_off, out_off;
+    // Pipelined host-buffer slides (rtdbs_batch_slide_pipelined): two device
+    // areas in the transfer area's layout, used by alternate slides, an I/O
+    // table and a captured graph for each, a copy stream each way beside the
+    // batch's stream, and per-area events ordering the three.
+    struct Pipe {
+        char *area[2] = {nullptr, nullptr};
+        size_t area_bytes = 0;
+        rtdbs::IO *d_io = nullptr;  // 2 * capacity entries: area 0's table, then area 1's
+        rtdbs::IO *h_io = nullptr;  // pinned staging of the two tables
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t gexec[2] = {nullptr, nullptr};
+        hipStream_t up = nullptr, down = nullptr;
+        hipEvent_t in_done[2] = {nullptr, nullptr}, kern_done[2] = {nullptr, nullptr},
+                   out_done[2] = {nullptr, nullptr};
+        bool ready = false;    // areas and tables match the members
+        int64_t issued = 0;    // slides enqueued since the areas were set up
+        bool waiting = false;  // the last slide's outputs are not yet waited for
+    } pipe;
     rtdbs::BatchArgs args{};
     rtdbs::LaunchShape shape{};
     hipStream_t hs = nullptr;
@@ -97,6 +115,11 @@ int batch_sync(rtdbs_batch *b);
 // Buffers in engine-pinned memory (rtdbs_host_alloc) are copied by DMA
 // directly, others through the batch's pinned staging.
 int batch_slide_host(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids);
+// Pipelined host-buffer slide (engine-pinned buffers): enqueues the slide and
+// returns once the previous pipelined slide's outputs are in its buffers.
+int batch_slide_pipelined(rtdbs_batch *b, const float *const *in, int32_t *const *labels, int32_t *const *cids);
+// Waits for the outputs of the last pipelined slide, if any.
+int pipe_drain(rtdbs_batch *b);
 // True if [p, p + bytes) lies inside one rtdbs_host_alloc allocation.
 bool host_pinned(const void *p, size_t bytes);
 int batch_collect_timing(rtdbs_batch *b);

@@ -99,6 +99,86 @@ def test_batch_pinned_contiguous_equals_pageable_and_plain(engine):
         s.close()
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_pipelined_slides_equal_plain(engine, use_graph):
+    # rtdbs_batch_slide_pipelined: each call returns once the previous call's
+    # outputs are in place; outputs alternate between two pinned sets so that
+    # a call's buffers stay untouched until the next call returns
+    R = engine
+    specs = [(2, 3000, 300, 0.8, 5), (3, 4000, 400, 0.9, 10), (2, 1200, 100, 0.6, 4), (3, 5000, 1000, 1.1, 20)]
+    slides = 6
+    datas = []
+    for i, (dim, W, S, eps, mp) in enumerate(specs):
+        n = W + (slides + 2) * S
+        datas.append(WL.blobs2d(n, seed=40 + i) if dim == 2 else _walks(n, 40 + i))
+
+    def streams():
+        out = []
+        for (dim, W, S, eps, mp), d in zip(specs, datas):
+            s = R.Stream(dim, W, S, eps, mp)
+            s.init(d[:W])
+            out.append(s)
+        return out
+
+    pipe, plain = streams(), streams()
+    bp = R.Batch(pipe, use_graph=use_graph)
+    in_bytes = sum(S * dim * 4 for dim, W, S, _, _ in specs)
+    out_bytes = sum(2 * W * 4 for _, W, _, _, _ in specs)
+    hin = R.HostBuffer(in_bytes * slides)
+    houts = [R.HostBuffer(out_bytes) for _ in range(2)]
+    ins = []
+    oi = 0
+    for t in range(slides):
+        row = []
+        for (dim, W, S, _, _), d in zip(specs, datas):
+            a = hin.array(np.float32, S * dim, offset=oi)
+            a[:] = d[W + t * S: W + (t + 1) * S].reshape(-1)
+            row.append(a)
+            oi += S * dim * 4
+        ins.append(row)
+    outs = []
+    for h in houts:
+        labs, cids, oo = [], [], 0
+        for _, W, _, _, _ in specs:
+            labs.append(h.array(np.int32, W, offset=oo))
+            cids.append(h.array(np.int32, W, offset=oo + W * 4))
+            oo += 2 * W * 4
+        outs.append((labs, cids))
+    expect = []
+    for t in range(slides):
+        expect.append([tuple(x.copy() for x in s.slide(d[W + t * S: W + (t + 1) * S]))
+                       for s, (dim, W, S, _, _), d in zip(plain, specs, datas)])
+
+    def check(t):
+        labs, cids = outs[t % 2]
+        for i in range(len(specs)):
+            assert np.array_equal(labs[i], expect[t][i][0]) and np.array_equal(cids[i], expect[t][i][1]), (t, i)
+
+    for t in range(slides):
+        labs, cids = outs[t % 2]
+        bp.slide_pipelined(ins[t], labs, cids)
+        if t >= 1:
+            check(t - 1)  # delivered when call t returned
+    bp.drain()
+    check(slides - 1)
+    # a synchronous slide after pipelined ones continues the same windows
+    t = slides
+    strides = [d[W + t * S: W + (t + 1) * S] for (dim, W, S, _, _), d in zip(specs, datas)]
+    lb, cb = bp.slide(strides)
+    for i, s in enumerate(plain):
+        lp, cp = s.slide(strides[i])
+        assert np.array_equal(lb[i], lp) and np.array_equal(cb[i], cp), i
+    # buffers outside engine-pinned memory are refused
+    with pytest.raises(R.RtdbsError):
+        bp.slide_pipelined([np.array(x) for x in ins[0]], *outs[0])
+    bp.close()
+    for s in pipe + plain:
+        s.close()
+    hin.free()
+    for h in houts:
+        h.free()
+
+
 def test_output_arrays_are_checked(engine):
     s = engine.Stream(2, 1000, 100, 0.8, 5)
     s.init(WL.blobs2d(1000, seed=1))
