@@ -1827,9 +1827,6 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
     const RowWalk w0 = row_walk(lane < V.R ? rw : 0ull, safeA ? cs : cs + nA);
     if (w0.rest == 0ull) return;
     const bool by_cells = safeA && uni(wave_sum(w0.len)) >= kCellLevelMin;
-    // a safe cell's element: its representative's slot, which every position
-    // of the cell holds in rep (unsafe cells: each core its own slot)
-    const int repA = uni(V.rep[cs]);
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1856,6 +1853,11 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
         arrived(y1);
         arrived(z1);
         arrived(r1);
+        // a safe cell's element: its representative's slot, which every
+        // position of the cell holds in rep - lane 0's query slot, so it comes
+        // with the queries instead of a round trip of its own (unsafe cells:
+        // each core its own slot, repA unused)
+        const int repA = lane_i(qslot, 0);
         const bool qcore = lane < nq && (denseA || (qf & kFlagCore));
         const uint64_t qm = __ballot(qcore);
         if (qm == 0ull) continue;
