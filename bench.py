@@ -755,7 +755,7 @@ def main():
     if (not only or only == "served") and not args.no_serve:
         log("served leg: %d streams x %d slides" % (args.serve_streams, args.serve_slides))
         out["served"] = served_leg(R, WL, args.serve_streams, args.serve_slides,
-                                   [float(x) for x in args.serve_rates.split(",")], 64, 2, 64)
+                                   [float(x) for x in args.serve_rates.split(",")], 64, 3, 64)
     print(json.dumps(out), flush=True)
 
 

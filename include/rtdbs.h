@@ -187,7 +187,10 @@ int rtdbs_autotune(const rtdbs_params *params, const float *points, int64_t n_po
 typedef struct rtdbs_server_config {
     size_t hbm_budget_bytes; /* admission limit on the sum of stream device bytes */
     int32_t max_batch;       /* most streams slid together in one launch */
-    int32_t lanes;           /* batches that may be on the device at once (HIP streams) */
+    int32_t lanes;           /* batches that may be on the device at once (HIP streams); 3 serves the
+                                most on one MI355X (one lane's transfers and host work overlap the
+                                others' kernels; a fourth stream shares one of the process's 4
+                                hardware queues) */
     int32_t use_graph;       /* capture recurring batch shapes as HIP graphs */
     int32_t admit_wait_ms;   /* open_stream waits this long for budget to free up */
     int32_t swap;            /* 1: streams beyond the budget live as host checkpoints and are

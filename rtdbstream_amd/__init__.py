@@ -520,7 +520,7 @@ class Batch:
 class Server:
     """Continuous-batching slide server with HBM-budget admission."""
 
-    def __init__(self, hbm_budget_bytes, max_batch=64, lanes=2, use_graph=False, admit_wait_ms=0, swap=False):
+    def __init__(self, hbm_budget_bytes, max_batch=64, lanes=3, use_graph=False, admit_wait_ms=0, swap=False):
         cfg = ServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_batch = int(max_batch)

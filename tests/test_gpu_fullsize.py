@@ -69,7 +69,7 @@ def test_256_served_streams_vs_validator(engine):
     slides = 2
     specs = WL.served_stream_specs(slides=slides)
     datas = [WL.stream_points(sp) for sp in specs]
-    srv = engine.Server(hbm_budget_bytes=64 << 30, max_batch=64, lanes=2)
+    srv = engine.Server(hbm_budget_bytes=64 << 30, max_batch=64, lanes=3)
     sids = [srv.open_stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"], d[:sp["window"]])
             for sp, d in zip(specs, datas)]
     # every request at once, interleaved across streams: the lanes form
