@@ -127,6 +127,7 @@ struct LaunchShape {
     int32_t count_cell;      // 1: cell-parallel range count (writes fcore and rep), 0: point-parallel
     int32_t union_cells;     // point-parallel path: 1 cell-level merge kernels, 0 the cell-parallel merge
     int32_t rank_split;      // 1: stride rank with 64-key blocks (small strides), 0: 256-key blocks
+    int32_t border_regroup;  // point-parallel outputs: 1 one lane per core, non-cores regrouped onto 4 lanes
 };
 
 enum Phase {

@@ -698,8 +698,14 @@ int refresh_batch(rtdbs_batch *b) {
     // the count takes 1 (64 streams: 238 vs 284 us; RTDBS_BORDER_LPQ=1|2|4
     // overrides)
     b->shape.lpq_border = env_int("RTDBS_BORDER_LPQ", 4, 1, 4);
-    if (n == 1 && b->streams[0]->p.lanes_per_query > 0)
+    // point-parallel outputs: one lane per core and four per non-core point
+    // (RTDBS_BORDER_REGROUP=0: lpq_border lanes for every position)
+    b->shape.border_regroup = env_int("RTDBS_BORDER_REGROUP", 1, 0, 1);
+    if (n == 1 && b->streams[0]->p.lanes_per_query > 0) {
+        // a single stream's own setting (the autotuner's point-parallel forms)
         b->shape.lpq = b->shape.lpq_border = b->streams[0]->p.lanes_per_query;
+        b->shape.border_regroup = 0;
+    }
     (void)cells_total;
     BatchArgs &a = b->args;
     std::memset(&a, 0, sizeof(a));

@@ -2240,6 +2240,93 @@ __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
 
 // First (minimum-slot) core of every cell - its union-find representative -
 // one thread per cell.
+// Border / noise, point-parallel with regrouping (large batches): one wave per
+// 64-position chunk, one lane per position. A core writes the root of its
+// element at once; the chunk's non-core points are then regrouped 16 at a time
+// onto four lanes each (a point's rank among the chunk's non-cores picks its
+// group, its query passed through LDS), which split its stencil rows as in
+// border_vblock. A core needs one lane and a non-core four, so a wave covers
+// four times the positions of the four-lanes-per-position form with about as
+// many round trips.
+struct BorderQ {
+    int c;
+    float x, y, z;
+    int slot;
+};
+
+template <int D>
+__device__ __forceinline__ void border_chunk(const BatchArgs &a, const ChunkCtx &x, int g, int s, const CellView &V,
+                                             BorderQ *lq) {
+    const int lane = lane_id();
+    const int p = x.p0 + lane;
+    const bool valid = p < x.p1;
+    // the query's flags, cell, coordinates and slot, one round trip
+    const int pp = valid ? p : x.p0;
+    const uint32_t po = (uint32_t)pp << 2;
+    const int qf = ldo(V.flags, (uint32_t)pp);
+    const int c = ldo(V.pcell, po);
+    const float qx = ldo(V.sx, po);
+    const float qy = D > 1 ? ldo(V.sy, po) : 0.f;
+    const float qz = D > 2 ? ldo(V.sz, po) : 0.f;
+    const uint64_t qk = ldo(V.skey, po << 1);
+    const int qr = ldo(V.rep, po);
+    arrived(c);
+    arrived(qx);
+    arrived(qy);
+    arrived(qz);
+    arrived(qk);
+    arrived(qr);
+    const int32_t *parent = ldc(a.s[s].parent);
+    const IO io = a.use_io1 ? a.io1 : a.io[s];
+    if (g == ldc(a.pre_chunk[s]) && lane == 0) {  // the stream's first block
+        StreamMut *m = ldc(a.s[s].mut);
+        if (a.stats) a.stats[(size_t)s * kNumStats + kStatCells] += (unsigned long long)m->ncell;
+        m->t = m->t + 1;
+        m->u = m->u + 1 == ldc(a.s[s].units) ? 0 : m->u + 1;
+    }
+    const int slot = (int)((uint32_t)qk & V.smask);
+    const bool core = valid && (qf & kFlagCore);
+    if (core) {
+        gl(io.label)[slot] = 0;
+        gl(io.cid)[slot] = uf_root(parent, qr);
+    }
+    const uint64_t nm = __ballot(valid && !core);
+    if (valid && !core) lq[__popcll(nm & ((1ull << lane) - 1ull))] = BorderQ{c, qx, qy, qz, slot};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nn = __popcll(nm);
+    const int grp = lane >> 2, sub = lane & 3;
+    int tests = 0;
+    for (int b0 = 0; b0 < nn; b0 += 16) {
+        const int i = b0 + grp;
+        const bool have = i < nn;
+        const BorderQ q = lq[have ? i : 0];
+        int best = INT_MAX;
+        if (have) best = border_walk<D>(V, parent, q.c, q.x, q.y, q.z, sub, 4, tests);
+        best = group_min<4>(best);
+        if (have && sub == 0) {
+            gl(io.label)[q.slot] = best != INT_MAX ? 1 : 2;
+            gl(io.cid)[q.slot] = best != INT_MAX ? best : q.slot;
+        }
+    }
+    if (a.stats) {
+        const unsigned long long t = (unsigned long long)wave_sum(tests);
+        if (lane == 0 && t) atomicAdd(a.stats + (size_t)s * kNumStats + kStatBorderTests, t);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_border_regroup(BatchArgs a) {
+    __shared__ BorderQ lq[64];
+    const int g = xcd_block(blockIdx.x, gridDim.x, a.xcd_groups & kXcdBorderPoints);
+    const ChunkCtx x = chunk_of(a, g);
+    const int s = uni(x.s);
+    const CellView V = cell_view(a.s[s]);
+    if (V.dim == 3) border_chunk<3>(a, x, g, s, V, lq);
+    else if (V.dim == 2) border_chunk<2>(a, x, g, s, V, lq);
+    else border_chunk<1>(a, x, g, s, V, lq);
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
     const int total = ldc(a.cell_pre[a.n]);
     for (int gc = blockIdx.x * kTileThreads + threadIdx.x; gc < total; gc += gridDim.x * kTileThreads) {
@@ -2814,6 +2901,7 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     PH_BEGIN(kPhBorder);
     if (sh.cellpar && a.stats) RTDBS_LAUNCH(k_border<WaveStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
     else if (sh.cellpar) RTDBS_LAUNCH(k_border<NoStats>, sh.cellpar_blocks, 64 * kCellWaves, a);
+    else if (sh.border_regroup) RTDBS_LAUNCH(k_border_regroup, sh.chunks, 64, a);
     else if (sh.lpq_border >= 4) RTDBS_LAUNCH(k_border_points<4>, sh.chunks * 4, 64, a);
     else if (sh.lpq_border == 2) RTDBS_LAUNCH(k_border_points<2>, sh.chunks * 2, 64, a);
     else RTDBS_LAUNCH(k_border_points<1>, sh.chunks, 64, a);

@@ -217,6 +217,25 @@ def test_xcd_grouped_work_order_agrees(engine, oracle, mode, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("regroup", ["0", "1"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_point_outputs_regrouped_agree(engine, oracle, dim, regroup, monkeypatch):
+    # point-parallel outputs with one lane per core and the non-core points
+    # regrouped onto four lanes each (1), or four lanes for every position (0);
+    # 30% noise puts more than 16 non-core points into some 64-position chunks
+    # (several regrouping rounds), and W = 3000 ends in a partial chunk
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", "2")
+    monkeypatch.setenv("RTDBS_BORDER_REGROUP", regroup)
+    data = _mixed(3000 + 4 * 300, dim, 47)
+    ref_l, ref_c = oracle.stream(data, 3000, 300, 0.3, 5, 4, method="grid")
+    s = engine.Stream(dim, 3000, 300, 0.3, 5)
+    s.init(data[:3000])
+    for t in range(4):
+        l, c = s.slide(data[3000 + t * 300: 3000 + (t + 1) * 300])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
 @pytest.mark.parametrize("lpq", [1, 2, 4])
 def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
     # lanes per query shape the point-parallel count and border kernels
