@@ -117,7 +117,6 @@ struct LaunchShape {
     int32_t stiles;   // total stride tiles
     int32_t wtiles;   // total window tiles
     int32_t chunks;   // total chunks
-    int32_t persist_blocks;  // 256-thread blocks for the persistent first-core kernel
     int32_t rows_blocks;     // 256-thread blocks for the row kernel
     int32_t cell_blocks;     // one-wave blocks for the persistent cell kernels
     int32_t lpq;             // lanes per query of the point-parallel count (1, 2 or 4)

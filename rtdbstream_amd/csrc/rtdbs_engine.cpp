@@ -653,9 +653,7 @@ int refresh_batch(rtdbs_batch *b) {
     const int rsplit = env_int("RTDBS_RANK_SPLIT", -1, 0, 1);
     b->shape.rank_split =
         rsplit >= 0 ? rsplit : ((long long)b->shape.stiles * (kTile / kTileThreads) < device_cus() ? 1 : 0);
-    const long long cap = (long long)device_cus() * 8;
-    long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
-    b->shape.persist_blocks = (int)std::max(1ll, std::min(cap, pb));
+    const long long pb = (rows_total + kTileThreads - 1) / kTileThreads;
     // a single stream's own tuned grid sizes (0: by launch size)
     const rtdbs_params *p1 = n == 1 ? &b->streams[0]->p : nullptr;
     // rows: a persistent grid of blocks per CU (measured best: 8 for a

@@ -2238,8 +2238,6 @@ __global__ __launch_bounds__(64) void k_border_points(BatchArgs a) {
     else border_vblock<LPQ, 1>(a, ql, vb, s, V);
 }
 
-// First (minimum-slot) core of every cell - its union-find representative -
-// one thread per cell.
 // Border / noise, point-parallel with regrouping (large batches): one wave per
 // 64-position chunk, one lane per position. A core writes the root of its
 // element at once; the chunk's non-core points are then regrouped 16 at a time
@@ -2327,44 +2325,51 @@ __global__ __launch_bounds__(64) void k_border_regroup(BatchArgs a) {
     else border_chunk<1>(a, x, g, s, V, lq);
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_fcore(BatchArgs a) {
-    const int total = ldc(a.cell_pre[a.n]);
-    for (int gc = blockIdx.x * kTileThreads + threadIdx.x; gc < total; gc += gridDim.x * kTileThreads) {
-        const CellCtx x = cell_of(a, gc);
-        const StreamDev &S = a.s[x.s];
-        const int c = x.c;
+// First core of every cell and union-find element of every position, in one
+// pass after the point-parallel count: one thread per position, a wave on 64
+// consecutive positions. Positions within a cell are in slot order, so a
+// cell's first core is its minimum-slot core: the lowest core flag of the
+// wave's ballot within the cell's lanes when the cell starts in this wave,
+// otherwise (it starts in an earlier wave, or holds no core in this one and
+// runs past it) a scan of its flags. The thread at the cell's first position
+// records fcore.
+__global__ __launch_bounds__(kTileThreads) void k_fcore_rep(BatchArgs a) {
+    const int g = blockIdx.x;
+    const int s = find_stream_c(a.pre_wtile, a.n, g);
+    const StreamDev &S = a.s[s];
+    const int W = ldc(S.W);
+    const int lane = lane_id();
+    const int p0 = (g - ldc(a.pre_wtile[s])) * kWTile;
+#pragma unroll
+    for (int r = 0; r < kWTile / kTileThreads; r++) {
+        const int p = p0 + r * kTileThreads + threadIdx.x;
+        const int wb = p - lane;  // the wave's first position
+        const bool v = p < W;
+        const int pp = v ? p : wb;
+        const int c = S.pcell[pp];
+        const int f = S.flags[pp];
+        const int cf = S.cflag[c];
         const int cs = S.cstart[c], ce = S.cstart[c + 1];
+        const uint64_t corem = __ballot(v && (f & kFlagCore));
         int fc = kNoCore;
-        if (S.cflag[c] & kCellDense) {
+        if (cf & kCellDense) {
             fc = cs;
         } else {
-            for (int i = cs; i < ce; i++)
+            int from = cs;  // positions still to scan: [from, ce)
+            if (cs >= wb) {
+                const int lo = cs - wb, hi = min(ce - wb, 64);
+                const uint64_t m = corem & (~0ull << lo) & (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull));
+                if (m) fc = wb + __ffsll((unsigned long long)m) - 1;
+                from = m ? ce : wb + 64;
+            }
+            for (int i = from; i < ce; i++)
                 if (S.flags[i] & kFlagCore) {
                     fc = i;
                     break;
                 }
         }
-        S.fcore[c] = fc;
-    }
-}
-
-// Union-find element of every position for the cell-parallel merge when the
-// point-parallel count ran (it does not know the cells' first cores): the
-// representative's slot in a safe cell, else the position's own slot.
-__global__ __launch_bounds__(kTileThreads) void k_rep(BatchArgs a) {
-    const int g = blockIdx.x;
-    const int s = find_stream_c(a.pre_wtile, a.n, g);
-    const StreamDev &S = a.s[s];
-    const int p0 = (g - ldc(a.pre_wtile[s])) * kWTile;
-#pragma unroll
-    for (int r = 0; r < kWTile / kTileThreads; r++) {
-        const int p = p0 + r * kTileThreads + threadIdx.x;
-        if (p < S.W) {
-            const int c = S.pcell[p];
-            const bool safe = S.cflag[c] & kCellSafe;
-            const int fc = S.fcore[c];
-            S.rep[p] = slot_of(S, safe && fc != kNoCore ? fc : p);
-        }
+        if (v && p == cs) S.fcore[c] = fc;
+        if (v) S.rep[p] = slot_of(S, (cf & kCellSafe) && fc != kNoCore ? fc : p);
     }
 }
 
@@ -2883,12 +2888,9 @@ hipError_t launch_slide(const BatchArgs &a, const LaunchShape &sh, hipStream_t s
     }
     PH_BEGIN(kPhUnion);
     // the cell-parallel count wrote every cell's first core and every
-    // position's union-find element; after the point-parallel count two small
-    // kernels do
-    if (!sh.count_cell) {
-        RTDBS_LAUNCH(k_fcore, sh.persist_blocks, kTileThreads, a);
-        RTDBS_LAUNCH(k_rep, sh.wtiles, kTileThreads, a);
-    }
+    // position's union-find element; after the point-parallel count one pass
+    // over the positions does
+    if (!sh.count_cell) RTDBS_LAUNCH(k_fcore_rep, sh.wtiles, kTileThreads, a);
     if (sh.union_cells) {
         RTDBS_LAUNCH(k_union_cells, sh.cell_blocks, 64, a);
         RTDBS_LAUNCH(k_union_points, sh.chunks, 64, a);

@@ -236,6 +236,23 @@ def test_point_outputs_regrouped_agree(engine, oracle, dim, regroup, monkeypatch
     s.close()
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_point_form_first_cores_agree(engine, oracle, k, monkeypatch):
+    # the point-parallel count's follow-up pass finds every cell's first core
+    # from its wave's core flags or, for a cell that crosses a 64-position
+    # wave, by scanning: dense blobs put well over 64 points into many cells
+    # (k = 1: side eps, no cell safe; k = 2: dense cells)
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", "2")
+    data = _mixed(6000 + 4 * 600, 3, 48, sigma=0.2)
+    ref_l, ref_c = oracle.stream(data, 6000, 600, 0.3, 20, 4, method="grid")
+    s = engine.Stream(3, 6000, 600, 0.3, 20, cell_div=k)
+    s.init(data[:6000])
+    for t in range(4):
+        l, c = s.slide(data[6000 + t * 600: 6000 + (t + 1) * 600])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
 @pytest.mark.parametrize("lpq", [1, 2, 4])
 def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
     # lanes per query shape the point-parallel count and border kernels
