@@ -2344,6 +2344,9 @@ __global__ __launch_bounds__(kTileThreads) void k_fcore_rep(BatchArgs a) {
     for (int r = 0; r < kWTile / kTileThreads; r++) {
         const int p = p0 + r * kTileThreads + threadIdx.x;
         const int wb = p - lane;  // the wave's first position
+        // a wave past the window's end (the last tile's tail) has no position;
+        // in the others the lanes past the end read the wave's first position
+        if (wb >= W) break;
         const bool v = p < W;
         const int pp = v ? p : wb;
         const int c = S.pcell[pp];

@@ -1,0 +1,88 @@
+"""Seeded random configurations against the oracle, through the C ABI.
+
+Each case draws the dimension, window, stride, Eps, MinPts, stencil reach k
+and kernel form from its seed, and the data from a mix of Gaussian blobs,
+uniform noise and repeated points, so that the cases land on the cell-level
+merge, the point-parallel regrouping of the count's and outputs' open
+queries, the one-pass first cores, cells that cross 64-position waves and
+strides from one point to the whole window. Bar: bit-exact `label` and `cid`
+on every slot of every slide.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("0", "1", "2")  # neighbour kernel form: by size, cell-parallel, point-parallel
+
+
+def _data(rng, n, dim):
+    spread = 6.0
+    nb = int(n * rng.uniform(0.4, 0.9))
+    centres = rng.uniform(0, spread, (int(rng.integers(2, 9)), dim))
+    sig = rng.uniform(0.05, 0.4)
+    p = centres[rng.integers(0, len(centres), nb)] + rng.normal(0, sig, (nb, dim))
+    q = rng.uniform(0, spread, (n - nb, dim))
+    pts = np.concatenate([p, q]).astype(np.float32)
+    pts = pts[rng.permutation(n)]
+    dup = rng.random(n) < 0.05  # some points repeat an earlier one (distance 0)
+    src = rng.integers(0, n, n)
+    pts[dup] = pts[src[dup]]
+    return pts
+
+
+def _case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    dim = int(rng.choice([1, 2, 3], p=[0.15, 0.35, 0.5]))
+    # the window is a whole number of strides (the ring's units)
+    units = int(rng.choice([1, 2, 5, 20, 300]))
+    S = max(1, int(np.exp(rng.uniform(np.log(100), np.log(6000)))) // units)
+    W = S * units
+    eps = float(np.float32(rng.uniform(0.05, 0.5)))
+    mp = int(rng.choice([1, 2, 4, 8, 16, 30]))
+    k = int(rng.choice([0, 1, 2, 3]))
+    mode = str(rng.choice(MODES))
+    slides = 3
+    data = _data(rng, W + slides * S, dim)
+    return dim, W, S, eps, mp, k, mode, slides, data
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_stream_bit_exact(engine, oracle, monkeypatch, seed):
+    dim, W, S, eps, mp, k, mode, slides, data = _case(seed)
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    ref_l, ref_c = oracle.stream(data, W, S, eps, mp, slides, method="grid")
+    s = engine.Stream(dim, W, S, eps, mp, cell_div=k)
+    s.init(data[:W])
+    for t in range(slides):
+        l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        assert np.array_equal(l, ref_l[t]), "seed %d (dim %d W %d S %d k %d mode %s) slide %d labels" % (
+            seed, dim, W, S, k, mode, t)
+        assert np.array_equal(c, ref_c[t]), "seed %d slide %d ids" % (seed, t)
+    s.close()
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("seed", range(3))
+def test_random_mixed_batch_bit_exact(engine, oracle, monkeypatch, mode, seed):
+    # six random streams of mixed dimension and reach in one launch per phase:
+    # the kernels pick each stream's arrays, dimension and stencil per lane
+    # group, cell or chunk
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    cases = [_case(100 + 10 * seed + i) for i in range(6)]
+    refs, streams = [], []
+    for dim, W, S, eps, mp, k, _, slides, data in cases:
+        refs.append(oracle.stream(data, W, S, eps, mp, slides, method="grid"))
+        st = engine.Stream(dim, W, S, eps, mp, cell_div=k)
+        st.init(data[:W])
+        streams.append(st)
+    b = engine.Batch(streams)
+    for t in range(3):
+        strides = [c[8][c[1] + t * c[2]: c[1] + (t + 1) * c[2]] for c in cases]
+        labels, cids = b.slide(strides)
+        for i, (l, c) in enumerate(zip(labels, cids)):
+            assert np.array_equal(l, refs[i][0][t]), "batch seed %d stream %d slide %d labels" % (seed, i, t)
+            assert np.array_equal(c, refs[i][1][t]), "batch seed %d stream %d slide %d ids" % (seed, i, t)
+    b.close()
+    for st in streams:
+        st.close()
