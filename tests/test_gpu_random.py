@@ -86,3 +86,22 @@ def test_random_mixed_batch_bit_exact(engine, oracle, monkeypatch, mode, seed):
     b.close()
     for st in streams:
         st.close()
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("W", [1, 2, 63, 64, 65, 511, 512, 513, 1025])
+def test_boundary_window_sizes_bit_exact(engine, oracle, monkeypatch, W, dim, mode):
+    # windows on either side of the 64-position waves and 512-position tiles
+    # that the kernels index by, strides of one point and of the whole window
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", mode)
+    rng = np.random.default_rng(7 * W + dim)
+    for S in sorted({1, W}):
+        data = _data(rng, W + 3 * S, dim)
+        ref_l, ref_c = oracle.stream(data, W, S, 0.3, 3, 3, method="grid")
+        s = engine.Stream(dim, W, S, 0.3, 3)
+        s.init(data[:W])
+        for t in range(3):
+            l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+            assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t]), "W %d S %d slide %d" % (W, S, t)
+        s.close()
