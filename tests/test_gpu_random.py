@@ -105,3 +105,39 @@ def test_boundary_window_sizes_bit_exact(engine, oracle, monkeypatch, W, dim, mo
             l, c = s.slide(data[W + t * S: W + (t + 1) * S])
             assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t]), "W %d S %d slide %d" % (W, S, t)
         s.close()
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_random_large_batch_bit_exact(engine, oracle, seed):
+    # eight random streams of 100-300 K points, 1-D to 3-D and k = 1 to 3, in
+    # one launch past 10^6 window points: the kernel set the engine picks by
+    # size (point-parallel count and regrouped outputs, one-pass first cores,
+    # XCD-ordered merge and outputs, 32 blocks per CU)
+    rng = np.random.default_rng(500 + seed)
+    cases, refs, streams = [], [], []
+    for i in range(8):
+        dim = int(rng.choice([1, 2, 3], p=[0.2, 0.4, 0.4]))
+        units = int(rng.choice([10, 100]))
+        S = int(rng.uniform(100_000, 300_000)) // units
+        W = S * units
+        # a few tens of points within eps at the blobs' density (k = 1 leaves
+        # no cell safe, so the merge unites every pair of nearby cores)
+        eps = float(np.float32(rng.uniform(*{1: (0.0005, 0.002), 2: (0.01, 0.03), 3: (0.03, 0.08)}[dim])))
+        mp = int(rng.choice([4, 10, 20]))
+        k = int(rng.choice([1, 2, 3], p=[0.2, 0.5, 0.3]))
+        data = _data(rng, W + 2 * S, dim)
+        cases.append((dim, W, S, data))
+        refs.append(oracle.stream(data, W, S, eps, mp, 2, method="grid"))
+        st = engine.Stream(dim, W, S, eps, mp, cell_div=k)
+        st.init(data[:W])
+        streams.append(st)
+    assert sum(c[1] for c in cases) > 1_000_000
+    b = engine.Batch(streams)
+    for t in range(2):
+        labels, cids = b.slide([c[3][c[1] + t * c[2]: c[1] + (t + 1) * c[2]] for c in cases])
+        for i, (l, c) in enumerate(zip(labels, cids)):
+            assert np.array_equal(l, refs[i][0][t]), "large batch seed %d stream %d slide %d labels" % (seed, i, t)
+            assert np.array_equal(c, refs[i][1][t]), "large batch seed %d stream %d slide %d ids" % (seed, i, t)
+    b.close()
+    for st in streams:
+        st.close()
