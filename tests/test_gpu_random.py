@@ -141,3 +141,47 @@ def test_random_large_batch_bit_exact(engine, oracle, seed):
     b.close()
     for st in streams:
         st.close()
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_random_server_mix_bit_exact(engine, oracle, swap):
+    # twelve random streams through the server from two submitting threads
+    # (each stream's requests from one of them, so they run in order), small
+    # batches so that the membership changes from launch to launch, and with
+    # swap=True a budget of half the streams' bytes, so that streams are
+    # evicted to host checkpoints and rebuilt between their requests
+    import threading
+    cases = [_case(200 + i) for i in range(12)]
+    refs = [oracle.stream(c[8], c[1], c[2], c[3], c[4], c[7], method="grid") for c in cases]
+    need = sum(engine.Stream.device_bytes(c[0], c[1], c[2], c[3], c[4], cell_div=c[5]) for c in cases)
+    budget = need // 2 if swap else 8 << 30
+    srv = engine.Server(hbm_budget_bytes=max(budget, max(
+        engine.Stream.device_bytes(c[0], c[1], c[2], c[3], c[4], cell_div=c[5]) for c in cases)),
+        max_batch=4, lanes=3, swap=swap)
+    sids = [srv.open_stream(c[0], c[1], c[2], c[3], c[4], c[8][:c[1]], cell_div=c[5]) for c in cases]
+    rng = np.random.default_rng(17 + int(swap))
+    order = [i for i in range(12) for _ in range(3)]
+    rng.shuffle(order)
+    tickets = {i: [] for i in range(12)}
+
+    def submit(part):
+        nxt = {}
+        for i in order:
+            if i % 2 != part:
+                continue
+            t = nxt.get(i, 0)
+            nxt[i] = t + 1
+            dim, W, S = cases[i][0], cases[i][1], cases[i][2]
+            tickets[i].append(srv.submit(sids[i], cases[i][8][W + t * S: W + (t + 1) * S]))
+
+    th = [threading.Thread(target=submit, args=(p,)) for p in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for i in range(12):
+        for t, tk in enumerate(tickets[i]):
+            l, c = srv.wait(tk, 60000)
+            assert np.array_equal(l, refs[i][0][t]), "server stream %d slide %d labels" % (i, t)
+            assert np.array_equal(c, refs[i][1][t]), "server stream %d slide %d ids" % (i, t)
+    srv.close()
