@@ -185,3 +185,49 @@ def test_random_server_mix_bit_exact(engine, oracle, swap):
             assert np.array_equal(l, refs[i][0][t]), "server stream %d slide %d labels" % (i, t)
             assert np.array_equal(c, refs[i][1][t]), "server stream %d slide %d ids" % (i, t)
     srv.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_random_pipelined_batch_bit_exact(engine, oracle, use_graph):
+    # six random streams of mixed dimension and reach slid by pipelined calls
+    # from engine-pinned buffers: every slide writes its own label and id
+    # arrays, which are complete once the next call (or drain) returns
+    cases = [_case(300 + i) for i in range(6)]
+    slides = 3
+    refs = [oracle.stream(c[8], c[1], c[2], c[3], c[4], slides, method="grid") for c in cases]
+    streams = []
+    for dim, W, S, eps, mp, k, _, _, data in cases:
+        st = engine.Stream(dim, W, S, eps, mp, cell_div=k)
+        st.init(data[:W])
+        streams.append(st)
+    b = engine.Batch(streams, use_graph=use_graph)
+    in_b = sum(c[2] * c[0] * 4 for c in cases) * slides
+    out_b = sum(c[1] * 8 for c in cases) * slides
+    hin, hout = engine.HostBuffer(in_b), engine.HostBuffer(out_b)
+    oi = oo = 0
+    ins, labs, cids = [], [], []
+    for t in range(slides):
+        row_i, row_l, row_c = [], [], []
+        for dim, W, S, _, _, _, _, _, data in cases:
+            a = hin.array(np.float32, S * dim, offset=oi)
+            a[:] = data[W + t * S: W + (t + 1) * S].reshape(-1)
+            row_i.append(a)
+            oi += S * dim * 4
+            row_l.append(hout.array(np.int32, W, offset=oo))
+            row_c.append(hout.array(np.int32, W, offset=oo + W * 4))
+            oo += W * 8
+        ins.append(row_i)
+        labs.append(row_l)
+        cids.append(row_c)
+    for t in range(slides):
+        b.slide_pipelined(ins[t], labs[t], cids[t])
+    b.drain()
+    for t in range(slides):
+        for i in range(len(cases)):
+            assert np.array_equal(labs[t][i], refs[i][0][t]), "pipelined stream %d slide %d labels" % (i, t)
+            assert np.array_equal(cids[t][i], refs[i][1][t]), "pipelined stream %d slide %d ids" % (i, t)
+    b.close()
+    for st in streams:
+        st.close()
+    hin.free()
+    hout.free()
