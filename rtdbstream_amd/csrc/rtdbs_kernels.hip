@@ -1687,8 +1687,12 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_border(BatchArgs a) {
 // the pairs of several candidate chunks and cells - of any streams of the
 // launch - are collected in LDS and each lane then walks the union-find of
 // its pair's stream, so the dependent loads of the walks overlap instead of
-// adding up cell after cell.
-constexpr int kEdgeCap = 256;
+// adding up cell after cell. 320 pairs (3.75 KB per wave) still let 8 blocks
+// of 4 waves share a CU's LDS; a fuller queue flushes less often where unites
+// are many (skewed3d 429 -> 419 us, a 62-stream served batch 1084 -> 1059 us,
+// 64 streams unchanged; 384 gained more on served batches but cost 1-2% at 64
+// streams and on walks3d).
+constexpr int kEdgeCap = 320;
 
 // One cell's header as the merge needs it: stream, cell within the stream,
 // first core position (kNoCore: none, or no cell), flags and position range.
