@@ -2043,43 +2043,55 @@ __global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu
 // adjacent and read the same candidates. D is the stream's dimension, so the
 // distance test and the loads carry no per-candidate dimension selects.
 //
-// One lane's share of a query's stencil: rows first, first + step, ...,
-// kRowGroup rows per round trip, kUnroll candidates per round trip. A group
-// of candidates is loaded whole, without clamping to its row's end: the loads
-// past the end read the next positions or the arrays' padding (kCandPad) and
-// are masked out of the count. The count is checked against min_pts once per
-// group.
+// Flat walks: a lane's rows, a group at a time, walked as one sequence of
+// candidates. The group's row words come in one round trip, then kFlat
+// candidates per round trip whatever the rows' lengths, so that a row holding
+// a cell or two does not cost a round trip of its own. Candidate t of a group
+// lies in the last row i with pre[i] <= t (pre: the candidates of the group's
+// earlier rows; an empty row shares its pre with the next row, which
+// overrides it), at position t + base[i].
+constexpr int kFlat = 8;        // candidates per round trip of a flat walk
+constexpr int kBorderRows = 8;  // rows per group of the outputs' walk (no early exit)
+
+// One lane's share of a query's stencil: rows first, first + step, ..., in
+// flat groups of kRowGroup rows, nearest rows first. The count is checked
+// against min_pts once per round trip.
 template <int D>
 __device__ __forceinline__ int count_walk(const CellView &V, int c, float qx, float qy, float qz, int first,
                                           int step, int &tests) {
-    const int R = V.R, minpts = V.min_pts;
+    const int minpts = V.min_pts;
     const float e2 = V.eps2;
     int cnt = 0;
-    for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
-        int rx[kRowGroup], ry[kRowGroup];
+    for (int r0 = first; r0 < V.R; r0 += kRowGroup * step) {
+        int base[kRowGroup], pre[kRowGroup];
+        int T = 0;
 #pragma unroll
         for (int i = 0; i < kRowGroup; i++) {
-            const uint64_t rw = (r0 + i * step < R) ? V.rows[(size_t)c * R + r0 + i * step] : 0ull;
-            rx[i] = (int)(uint32_t)rw;
-            ry[i] = (int)(uint32_t)(rw >> 32);
+            const uint64_t rw = (r0 + i * step < V.R) ? V.rows[(size_t)c * V.R + r0 + i * step] : 0ull;
+            const int rx = (int)(uint32_t)rw, ry = (int)(uint32_t)(rw >> 32);  // rx < ry, or both 0
+            pre[i] = T;
+            base[i] = rx - T;
+            T += ry - rx;
         }
+        for (int t0 = 0; t0 < T; t0 += kFlat) {
+            float cx[kFlat], cy[kFlat], cz[kFlat];
 #pragma unroll
-        for (int i = 0; i < kRowGroup; i++) {
-            for (int j = rx[i]; j < ry[i]; j += kUnroll) {
-                const uint32_t o = (uint32_t)j << 2;
-                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+            for (int u = 0; u < kFlat; u++) {
+                // clamped to the last candidate: every load reads a real one
+                const int t = min(t0 + u, T - 1);
+                int b = base[0];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    cx[u] = ldo(V.sx, o + 4u * u);
-                    cy[u] = D > 1 ? ldo(V.sy, o + 4u * u) : 0.f;
-                    cz[u] = D > 2 ? ldo(V.sz, o + 4u * u) : 0.f;
-                }
-                const int nv = ry[i] - j;
-                tests += min(nv, kUnroll);
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++) cnt += (u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2);
-                if (cnt >= minpts) return cnt;
+                for (int i = 1; i < kRowGroup; i++) b = t >= pre[i] ? base[i] : b;
+                const uint32_t o = (uint32_t)(t + b) << 2;
+                cx[u] = ldo(V.sx, o);
+                cy[u] = D > 1 ? ldo(V.sy, o) : 0.f;
+                cz[u] = D > 2 ? ldo(V.sz, o) : 0.f;
             }
+            const int nv = T - t0;
+            tests += min(nv, kFlat);
+#pragma unroll
+            for (int u = 0; u < kFlat; u++) cnt += (u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2);
+            if (cnt >= minpts) return cnt;
         }
     }
     return cnt;
@@ -2141,42 +2153,48 @@ __global__ __launch_bounds__(64) void k_count_points(BatchArgs a) {
 // stream's ring cursor, as the cell-parallel output kernel does. D is the
 // stream's dimension.
 //
-// One lane's share of a non-core point's stencil, walked as count_walk walks
-// it (groups loaded past a row's end, masked), without an early exit: the
-// minimum root of the cores within eps.
+// One lane's share of a non-core point's stencil in flat groups of
+// kBorderRows rows (larger than the count's: no early exit here), without an
+// early exit: the minimum root of the cores within eps. A candidate's flags
+// and element load only when it is within eps.
 template <int D>
 __device__ __forceinline__ int border_walk(const CellView &V, const int32_t *parent, int c, float qx, float qy,
                                            float qz, int first, int step, int &tests) {
-    const int R = V.R;
     const float e2 = V.eps2;
     int best = INT_MAX;
-    for (int r0 = first; r0 < R; r0 += kRowGroup * step) {
-        int rx[kRowGroup], ry[kRowGroup];
+    for (int r0 = first; r0 < V.R; r0 += kBorderRows * step) {
+        int base[kBorderRows], pre[kBorderRows];
+        int T = 0;
 #pragma unroll
-        for (int i = 0; i < kRowGroup; i++) {
-            const uint64_t rw = (r0 + i * step < R) ? V.rows[(size_t)c * R + r0 + i * step] : 0ull;
-            rx[i] = (int)(uint32_t)rw;
-            ry[i] = (int)(uint32_t)(rw >> 32);
+        for (int i = 0; i < kBorderRows; i++) {
+            const uint64_t rw = (r0 + i * step < V.R) ? V.rows[(size_t)c * V.R + r0 + i * step] : 0ull;
+            const int rx = (int)(uint32_t)rw, ry = (int)(uint32_t)(rw >> 32);
+            pre[i] = T;
+            base[i] = rx - T;
+            T += ry - rx;
         }
+        for (int t0 = 0; t0 < T; t0 += kFlat) {
+            float cx[kFlat], cy[kFlat], cz[kFlat];
+            int pos[kFlat];
 #pragma unroll
-        for (int i = 0; i < kRowGroup; i++) {
-            for (int j = rx[i]; j < ry[i]; j += kUnroll) {
-                const uint32_t o = (uint32_t)j << 2;
-                float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+            for (int u = 0; u < kFlat; u++) {
+                const int t = min(t0 + u, T - 1);
+                int b = base[0];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
-                    cx[u] = ldo(V.sx, o + 4u * u);
-                    cy[u] = D > 1 ? ldo(V.sy, o + 4u * u) : 0.f;
-                    cz[u] = D > 2 ? ldo(V.sz, o + 4u * u) : 0.f;
-                }
-                const int nv = ry[i] - j;
-                tests += min(nv, kUnroll);
-#pragma unroll
-                for (int u = 0; u < kUnroll; u++)
-                    if ((u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2) &&
-                        (ldo(V.flags, (uint32_t)(j + u)) & kFlagCore))
-                        best = min(best, uf_root(parent, ldo(V.rep, o + 4u * u)));
+                for (int i = 1; i < kBorderRows; i++) b = t >= pre[i] ? base[i] : b;
+                pos[u] = t + b;
+                const uint32_t o = (uint32_t)pos[u] << 2;
+                cx[u] = ldo(V.sx, o);
+                cy[u] = D > 1 ? ldo(V.sy, o) : 0.f;
+                cz[u] = D > 2 ? ldo(V.sz, o) : 0.f;
             }
+            const int nv = T - t0;
+            tests += min(nv, kFlat);
+#pragma unroll
+            for (int u = 0; u < kFlat; u++)
+                if ((u < nv) & (sqd(D, qx, qy, qz, cx[u], cy[u], cz[u]) < e2) &&
+                    (ldo(V.flags, (uint32_t)pos[u]) & kFlagCore))
+                    best = min(best, uf_root(parent, ldo(V.rep, (uint32_t)pos[u] << 2)));
         }
     }
     return best;
