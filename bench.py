@@ -474,7 +474,9 @@ def plain_leg(R, WL, name, steps, warmup, autotune=True, roof_sample=4):
                                         "rows_per_cu", "from_cache", "trials")},
         "tune_s": round(tune_s, 2),
         "cells_last": info["cells"],
-        "roofline": roofline_obj(B, ms / max(1, n), "k_count", "k_count_%s" % name),
+        "roofline": roofline_obj(B, ms / max(1, n),
+                                 "k_count" if tu["lanes_per_query"] == 0 else "k_count_points<%d>" % tu["lanes_per_query"],
+                                 "k_count_%s" % name),
     }
 
 
