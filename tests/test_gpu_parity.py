@@ -267,6 +267,35 @@ def test_lanes_per_query_settings_agree(engine, oracle, lpq, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("lpq", [1, 2, 4])
+@pytest.mark.parametrize("case", ["long1d", "dense2d", "sparse3d"])
+def test_flat_walks_agree(engine, oracle, case, lpq, monkeypatch):
+    # the point-parallel count and outputs walk a group of stencil rows as one
+    # sequence, 8 candidates per round across row boundaries: rows of dozens
+    # of candidates (1-D, many rounds per row), high MinPts on dense blobs
+    # (every group of rows walked), and sparse 3-D data (most rows empty,
+    # groups without a candidate)
+    monkeypatch.setenv("RTDBS_NEIGHBOR_MODE", "2")
+    W, S = 3000, 300
+    if case == "long1d":
+        dim, eps, mp = 1, 0.05, 30
+        data = np.random.default_rng(61).uniform(0, 10, (W + 4 * S, 1)).astype(np.float32)
+    elif case == "dense2d":
+        dim, eps, mp = 2, 0.3, 40
+        data = _mixed(W + 4 * S, 2, 62, sigma=0.4, noise=0.2)
+    else:
+        dim, eps, mp = 3, 0.5, 4
+        data = np.random.default_rng(63).uniform(0, 12, (W + 4 * S, 3)).astype(np.float32)
+    ref_l, ref_c = oracle.stream(data, W, S, eps, mp, 4, method="grid")
+    assert 0 < int(np.sum(ref_l[-1] == 0)) < W  # cores and non-cores both
+    s = engine.Stream(dim, W, S, eps, mp, lanes_per_query=lpq)
+    s.init(data[:W])
+    for t in range(4):
+        l, c = s.slide(data[W + t * S: W + (t + 1) * S])
+        assert np.array_equal(l, ref_l[t]) and np.array_equal(c, ref_c[t])
+    s.close()
+
+
 @pytest.mark.parametrize("mode", ["1", "2", "2u"])
 @pytest.mark.parametrize("dim,W,S,eps,mp,k,seed", [SMALL[2], SMALL[6], SMALL[9], SMALL[13]])
 def test_neighbour_kernel_forms_agree(engine, oracle, monkeypatch, mode, dim, W, S, eps, mp, k, seed):
