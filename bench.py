@@ -610,32 +610,50 @@ def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, s
     for f, budget, swap in runs:
         if not swap and behind >= 2:
             continue
-        srv = R.Server(hbm_budget_bytes=budget, max_batch=max_batch, lanes=lanes, swap=swap)
-        sids = [srv.open_stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"],
-                                datas[i][:sp["window"]]) for i, sp in enumerate(specs)]
         order = np.repeat(np.arange(n_streams), slides)
         rng.shuffle(order)
         gaps = rng.exponential(1.0 / (f * capacity), size=len(order))
-        nxt = [0] * n_streams
-        tickets = []
-        t0 = time.perf_counter()
-        due = t0
-        for k, i in enumerate(order):
-            due += gaps[k]
-            while True:
-                now = time.perf_counter()
-                if now >= due:
-                    break
-                if due - now > 2e-4:
-                    time.sleep((due - now) * 0.5)
-            t = nxt[i]
-            nxt[i] += 1
-            tickets.append(srv.submit(sids[i], stride(i, t), labs[i], cids[i]))
-        for tk in tickets:
-            srv.wait(tk, 120000)
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-        srv.close()
+        srv = R.Server(hbm_budget_bytes=budget, max_batch=max_batch, lanes=lanes, swap=swap)
+        failed = []
+        try:
+            sids = [srv.open_stream(sp["dim"], sp["window"], sp["stride"], sp["eps"], sp["min_pts"],
+                                    datas[i][:sp["window"]]) for i, sp in enumerate(specs)]
+            nxt = [0] * n_streams
+            tickets = []
+            t0 = time.perf_counter()
+            due = t0
+            for k, i in enumerate(order):
+                due += gaps[k]
+                while True:
+                    now = time.perf_counter()
+                    if now >= due:
+                        break
+                    if due - now > 2e-4:
+                        time.sleep((due - now) * 0.5)
+                t = nxt[i]
+                nxt[i] += 1
+                tickets.append(srv.submit(sids[i], stride(i, t), labs[i], cids[i]))
+            for tk in tickets:
+                try:
+                    srv.wait(tk, 120000)
+                except R.RtdbsError as e:  # a failed request is reported, not fatal
+                    failed.append(str(e))
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        except R.RtdbsError as e:  # opening or submitting failed: the run is reported
+            failed.append(str(e))
+        finally:
+            srv.close()
+        if failed:
+            row = {"rate_factor": f, "requests": len(order), "failed": len(failed), "error": failed[0]}
+            log("served f=%.2f%s: %d of %d requests failed, first: %s" %
+                (f, " (swapping)" if swap else "", len(failed), len(order), failed[0]))
+            if swap:
+                out["swap_run"] = row
+            else:
+                out["sweep"].append(row)
+                behind = 2  # no higher rates after a failure
+            continue
         for i in range(n_streams):
             exact = exact and np.array_equal(labs[i], last[i][0]) and np.array_equal(cids[i], last[i][1])
         nreq = len(order)
@@ -654,10 +672,11 @@ def served_leg(R, WL, n_streams, slides, factors, max_batch, lanes, budget_gb, s
         log("served f=%.2f: %s" % (f, row))
         if rate < 0.9 * f * capacity:
             behind += 1
-    sat = max(out["sweep"], key=lambda r: r["slides_per_s"])
-    out["capacity_slides_per_s"] = sat["slides_per_s"]
-    out["capacity_points_per_s"] = sat["points_per_s"]
-    out["capacity_vs_plain"] = round(sat["slides_per_s"] / capacity, 3)
+    ok = [r for r in out["sweep"] if "error" not in r]
+    sat = max(ok, key=lambda r: r["slides_per_s"]) if ok else None
+    out["capacity_slides_per_s"] = sat["slides_per_s"] if sat else None
+    out["capacity_points_per_s"] = sat["points_per_s"] if sat else None
+    out["capacity_vs_plain"] = round(sat["slides_per_s"] / capacity, 3) if sat else None
     out["labels_equal_plain"] = bool(exact)
     hin.free()
     hout.free()
@@ -756,8 +775,12 @@ def main():
         log("cpu baseline: %s" % out["cpu_baseline"])
     if (not only or only == "served") and not args.no_serve:
         log("served leg: %d streams x %d slides" % (args.serve_streams, args.serve_slides))
-        out["served"] = served_leg(R, WL, args.serve_streams, args.serve_slides,
-                                   [float(x) for x in args.serve_rates.split(",")], 64, 3, 64)
+        try:
+            out["served"] = served_leg(R, WL, args.serve_streams, args.serve_slides,
+                                       [float(x) for x in args.serve_rates.split(",")], 64, 3, 64)
+        except Exception as e:  # reported in the line; the other legs' figures stand
+            out["served"] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("served leg failed: %s" % out["served"]["error"])
     print(json.dumps(out), flush=True)
 
 

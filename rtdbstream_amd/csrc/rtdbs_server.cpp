@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct Request {
     int32_t *cid = nullptr;
     int state = kQueued;
     int status = RTDBS_OK;
+    std::string error;  // the failing call's message when status != RTDBS_OK
     Clock::time_point t_submit, t_done;
 };
 
@@ -189,6 +191,7 @@ bool swap_in_one(rtdbs_server *srv, std::unique_lock<std::mutex> &lk) {
         srv->arrivals.erase(std::remove(srv->arrivals.begin(), srv->arrivals.end(), r), srv->arrivals.end());
         r->state = kDone;
         r->status = rc;
+        r->error = rtdbs_last_error();  // this thread's message of the failing call
         r->t_done = Clock::now();
         srv->failed++;
     }
@@ -246,6 +249,7 @@ void lane_main(rtdbs_server *srv, int lane) {
             rc = batch_slide_host(b, ins.data(), labels.data(), cids.data());
         }
         const auto now = Clock::now();
+        const std::string err = rc == RTDBS_OK ? std::string() : std::string(rtdbs_last_error());
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             srv->batches++;
@@ -258,6 +262,7 @@ void lane_main(rtdbs_server *srv, int lane) {
                 ss->last_used = now_clock;
                 r->state = kDone;
                 r->status = rc;
+                r->error = err;
                 r->t_done = now;
                 if (rc == RTDBS_OK) {
                     srv->completed++;
@@ -533,9 +538,10 @@ extern "C" int rtdbs_server_wait(rtdbs_server *srv, int64_t ticket, int32_t time
         return fail(RTDBS_ETIMEOUT, "ticket %lld not done after %d ms", (long long)ticket, timeout_ms);
     }
     const int st = r->status;
+    const std::string err = r->error;
     srv->tickets.erase(it);
     delete r;
-    if (st) return fail(st, "request failed with status %d", st);
+    if (st) return fail(st, "request failed with status %d%s%s", st, err.empty() ? "" : ": ", err.c_str());
     return RTDBS_OK;
 }
 
@@ -547,9 +553,10 @@ extern "C" int rtdbs_server_poll(rtdbs_server *srv, int64_t ticket) {
     Request *r = it->second;
     if (r->state != kDone) return 0;
     const int st = r->status;
+    const std::string err = r->error;
     srv->tickets.erase(it);
     delete r;
-    if (st) return fail(st, "request failed with status %d", st);
+    if (st) return fail(st, "request failed with status %d%s%s", st, err.empty() ? "" : ": ", err.c_str());
     return 1;
 }
 

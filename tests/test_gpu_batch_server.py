@@ -277,6 +277,34 @@ def test_server_evict_readmit_and_cancel(engine):
     srv.close()
 
 
+def test_server_failed_swap_in_reports_its_cause(engine):
+    """A swap-in whose device allocation fails fails only the request that
+    asked for it, and the error names the call that failed (not only the
+    status); the stream's next request swaps it in and slides as if the
+    failed request had never been made."""
+    R = engine
+    specs = _specs()[:2]
+    datas = [_data(sp, 2) for sp in specs]
+    sizes = [R.Stream.device_bytes(sp["dim"], sp["W"], sp["S"], sp["eps"], sp["mp"], cell_div=sp["k"])
+             for sp in specs]
+    srv = R.Server(hbm_budget_bytes=max(sizes), max_batch=4, lanes=1, swap=True)  # one stream at a time
+    sids = [srv.open_stream(sp["dim"], sp["W"], sp["S"], sp["eps"], sp["mp"], d[:sp["W"]], cell_div=sp["k"])
+            for sp, d in zip(specs, datas)]
+    srv.wait(srv.submit(sids[0], _stride(specs[0], datas[0], 0)), 30000)  # stream 0 on the device
+    R.lib().rtdbs_test_fail_allocations(1)
+    with pytest.raises(R.RtdbsError) as e:
+        srv.wait(srv.submit(sids[1], _stride(specs[1], datas[1], 0)), 30000)
+    assert e.value.code == R.ENOMEM
+    assert "allocation hook" in str(e.value)
+    assert srv.stats()["failed"] == 1
+    l, c = srv.wait(srv.submit(sids[1], _stride(specs[1], datas[1], 0)), 30000)
+    s = _stream(R, specs[1], datas[1])
+    l2, c2 = s.slide(_stride(specs[1], datas[1], 0))
+    assert np.array_equal(l, l2) and np.array_equal(c, c2)
+    s.close()
+    srv.close()
+
+
 def test_server_swaps_under_a_binding_budget(engine):
     """A budget that holds about half the streams: with swapping on, every
     stream still opens, requests of streams not on the device swap them in
