@@ -1402,84 +1402,6 @@ __device__ __forceinline__ int walk_chunk(RowWalk &w, int jb, int *end) {
     return base + min(j, *end - 1);
 }
 
-// Ranges held one per lane (stencil rows, candidate cells) as one candidate
-// sequence of known length: lane r holds range r's first candidate index pre
-// (the candidates of the lanes below) and base = first position - pre; its
-// end is the next lane's pre. Stateless per chunk: the lane holding candidate
-// j is found through a per-wave LDS row of 64 slots, each tagged with the
-// call that wrote it (no clearing pass).
-struct RowScan {
-    int base, pre;  // per lane
-    int total;      // uniform
-};
-
-// Inclusive prefix sum over the wave: row shifts within each 16-lane row
-// (invalid sources read 0), then the totals of the rows below added.
-__device__ __forceinline__ int wave_incl_sum(int v) {
-    const int lane = lane_id();
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);  // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);  // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);  // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);  // row_shr:8
-    const int s0 = __builtin_amdgcn_readlane(v, 15);
-    const int s1 = __builtin_amdgcn_readlane(v, 31);
-    const int s2 = __builtin_amdgcn_readlane(v, 47);
-    v += (lane >= 16 ? s0 : 0) + (lane >= 32 ? s1 : 0) + (lane >= 48 ? s2 : 0);
-    return v;
-}
-
-// Range [first, first + len) in each lane (len >= 0).
-__device__ __forceinline__ RowScan row_scan(int first, int len) {
-    RowScan w;
-    const int incl = wave_incl_sum(len);
-    w.pre = incl - len;
-    w.base = first - w.pre;
-    w.total = __builtin_amdgcn_readlane(incl, 63);
-    return w;
-}
-
-// The rows (row_word), each cut at window position `limit`.
-__device__ __forceinline__ RowScan row_scan_rows(uint64_t rw, int limit) {
-    const int rx = (int)(uint32_t)rw;
-    return row_scan(rx, max(0, min((int)(uint32_t)(rw >> 32), limit) - rx));
-}
-
-// End of each lane's range: the next lane's pre (lane 63: the total).
-__device__ __forceinline__ int scan_end(const RowScan &w) {
-    const int nx = __builtin_amdgcn_update_dpp(0, w.pre, 0x130, 0xF, 0xF, false);  // wave_shl:1, from lane + 1
-    return lane_id() == 63 ? w.total : nx;
-}
-
-// Per-wave LDS slots of the scan walk and the tag of the last call. The merge
-// kernel zeroes the slots and the tag at its start; a wave's calls in one
-// launch (one per candidate chunk) stay far below the 2^32 tags.
-struct ScanLds {
-    uint64_t *slot;  // 64 slots: (tag << 32) | base
-    uint32_t tag;    // uniform
-};
-
-// This lane's candidate position in chunk [jb, jb + 64) of w (lanes past the
-// last candidate get the last one; 0 when jb >= total, which callers check
-// before any use).
-__device__ __forceinline__ int scan_chunk(const RowScan &w, int jb, ScanLds &L) {
-    const int lane = lane_id();
-    const uint32_t tag = ++L.tag;
-    const int end = scan_end(w);
-    // the range holding candidate jb takes slot 0, a range starting inside
-    // the chunk its own offset: distinct slots
-    if (end > w.pre && end > jb && w.pre < jb + 64)
-        L.slot[max(w.pre - jb, 0)] = ((uint64_t)tag << 32) | (uint32_t)w.base;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint64_t v = L.slot[lane];
-    const uint64_t m = __ballot((uint32_t)(v >> 32) == tag) & ((2ull << lane) - 1ull);
-    const int src = m ? 63 - __clzll((long long)m) : 0;
-    const int b = __shfl((int)(uint32_t)v, src);
-    __builtin_amdgcn_wave_barrier();
-    return jb < w.total ? b + min(jb + lane, w.total - 1) : 0;
-}
-
 // A cell's header as each lane loaded it: run start and end, class, and
 // stencil row `lane` (lanes < R). One round trip, issued ahead of its use.
 struct CellHdr {
@@ -1837,9 +1759,8 @@ struct Edge {
 };
 
 struct EdgeBuf {
-    Edge *e;     // this wave's LDS slice
-    int n;       // pairs held (uniform)
-    ScanLds sl;  // this wave's slots of the candidate walks
+    Edge *e;  // this wave's LDS slice
+    int n;    // pairs held (uniform)
 };
 
 __device__ __forceinline__ void edge_flush(const BatchArgs &a, EdgeBuf &eb) {
@@ -1907,9 +1828,9 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
     // further chunk is a round of its own
     const bool safeA = h.cf & kCellSafe, denseA = h.cf & kCellDense;
     const int cs = h.cs, nA = h.ce - h.cs;
-    const RowScan w0 = row_scan_rows(lane < V.R ? rw : 0ull, safeA ? cs : cs + nA);
-    if (w0.total == 0) return;
-    const bool by_cells = safeA && w0.total >= kCellLevelMin;
+    const RowWalk w0 = row_walk(lane < V.R ? rw : 0ull, safeA ? cs : cs + nA);
+    if (w0.rest == 0ull) return;
+    const bool by_cells = safeA && uni(wave_sum(w0.len)) >= kCellLevelMin;
     for (int q0 = 0; q0 < nA; q0 += 64) {
         const int nq = min(64, nA - q0);
         const int pq = cs + q0 + min(lane, nq - 1);
@@ -1920,8 +1841,9 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
         const int qslot = V.rep[pq];  // read for the cores of unsafe cells: the point's own slot
         // the point scan's first chunk, in the same round (not for the
         // cell-level pass, which scans other walks)
-        const int end1 = w0.total;
-        const int pos1 = scan_chunk(w0, 0, eb.sl);
+        RowWalk w1 = w0;
+        int end1 = 0;
+        const int pos1 = walk_chunk(w1, 0, &end1);
         const int f1 = by_cells ? 0 : V.flags[pos1];
         const float x1 = by_cells ? 0.f : V.sx[pos1];
         const float y1 = by_cells || dim < 2 ? 0.f : V.sy[pos1];
@@ -1988,10 +1910,11 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
                 }
         };
         // the cores qm of this block against the candidate positions of walk w
-        auto scan = [&](const RowScan &w) __attribute__((always_inline)) {
-            const int end = w.total;
-            for (int jb = 0; jb < end; jb += 64) {
-                const int pos = scan_chunk(w, jb, eb.sl);
+        auto scan = [&](RowWalk w) __attribute__((always_inline)) {
+            for (int jb = 0;; jb += 64) {
+                int end;
+                const int pos = walk_chunk(w, jb, &end);
+                if (jb >= end) break;
                 const int f = V.flags[pos];
                 const float cx = V.sx[pos];
                 const float cy = dim > 1 ? V.sy[pos] : 0.f;
@@ -2007,9 +1930,10 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
         if (!by_cells) {
             // the first chunk came with the queries
             if (end1 > 0) test(0, end1, pos1, f1, x1, y1, z1, r1);
-            for (int jb = 64; jb < end1; jb += 64) {
-                const int end = end1;
-                const int pos = scan_chunk(w0, jb, eb.sl);
+            for (int jb = 64;; jb += 64) {
+                int end;
+                const int pos = walk_chunk(w1, jb, &end);
+                if (jb >= end) break;
                 const int f = V.flags[pos];
                 const float cx = V.sx[pos];
                 const float cy = dim > 1 ? V.sy[pos] : 0.f;
@@ -2025,13 +1949,16 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
         }
         // candidate cells: row r (lane r) covers positions [rx, rx + len), all
         // before cs, i.e. the cells pcell[rx] .. pcell[rx + len - 1] (< A)
-        const int wlen = scan_end(w0) - w0.pre, wrx = w0.base + w0.pre;
-        const int clo = wlen > 0 ? V.pcell[wrx] : 0;
-        const int chi = wlen > 0 ? V.pcell[wrx + wlen - 1] + 1 : 0;
-        const RowScan cw = row_scan(clo, chi - clo);
-        const int endc = cw.total;
-        for (int jb = 0; jb < endc; jb += 64) {
-            const int B = scan_chunk(cw, jb, eb.sl);
+        const int clo = w0.len > 0 ? V.pcell[w0.rx] : 0;
+        const int chi = w0.len > 0 ? V.pcell[w0.rx + w0.len - 1] + 1 : 0;
+        RowWalk cw;
+        cw.rx = clo;
+        cw.len = chi - clo;
+        cw.rest = __ballot(cw.len > 0);
+        for (int jb = 0;; jb += 64) {
+            int endc;
+            const int B = walk_chunk(cw, jb, &endc);
+            if (jb >= endc) break;
             const bool valid = jb + lane < endc;
             const int fb = V.fcore[B];
             const int cfb = V.cflag[B];
@@ -2061,8 +1988,11 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
             // a safe cell's element is its first core's slot; one lane per cell
             unite_once_per_rep(a, eb, s, (hit >> lane) & 1ull, bk, repA, st);
             // the cells with cores not settled: scan their points
-            const RowScan pw = row_scan(bs, (has && !((hit >> lane) & 1ull)) ? be - bs : 0);
-            if (pw.total) scan(pw);
+            RowWalk pw;
+            pw.rx = bs;
+            pw.len = (has && !((hit >> lane) & 1ull)) ? be - bs : 0;
+            pw.rest = __ballot(pw.len > 0);
+            if (pw.rest) scan(pw);
         }
     }
 }
@@ -2070,12 +2000,10 @@ __device__ void union_cell(const BatchArgs &a, int s, const CellView &V, const C
 template <class ST>
 __global__ __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_union(BatchArgs a) {
     __shared__ Edge edges[kCellWaves][kEdgeCap];
-    __shared__ uint64_t lslots[kCellWaves][64];
     const int total = uni(ldc(a.cell_pre[a.n]));
     const int nw = gridDim.x * kCellWaves;
     ST st;
-    lslots[threadIdx.x >> 6][threadIdx.x & 63] = 0ull;  // tag 0: never written by a call
-    EdgeBuf eb{edges[threadIdx.x >> 6], 0, ScanLds{lslots[threadIdx.x >> 6], 0u}};
+    EdgeBuf eb{edges[threadIdx.x >> 6], 0};
     const int w = uni(xcd_block(blockIdx.x, gridDim.x, a.xcd_groups & kXcdUnion) * kCellWaves + (threadIdx.x >> 6));
     // stencil rows of a cell, lane r: row r (every lane loads, clamped to the
     // last row; lanes >= R are cleared where the rows are used)
