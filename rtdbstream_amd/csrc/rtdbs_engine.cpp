@@ -137,10 +137,10 @@ static Layout make_layout(const rtdbs_params &p, int R) {
     const size_t ntile = (W + kWTile - 1) / kWTile;
     L.off_mut = take(sizeof(StreamMut));
     L.off_skey = take(W * 8);
-    // window coordinates (both buffers of the ping-pong); every candidate
-    // walk reads real candidates only (a round past a row's or a group's end
-    // repeats the last one)
-    const size_t wc = W * 4;
+    // window coordinates (both buffers of the ping-pong) with kCandPad
+    // readable floats past the end: the point-parallel kernels load candidates
+    // in groups that may run past a stencil row's end
+    const size_t wc = (W + kCandPad) * 4;
     L.off_sx = take(wc);
     L.off_sy = d > 1 ? take(wc) : 0;
     L.off_sz = d > 2 ? take(wc) : 0;
