@@ -159,7 +159,8 @@ void rtdbs_batch_destroy(rtdbs_batch *b);
  * dbscan.cpp:34). Times trial slides of the given stream sample for every
  * stencil reach k (grid side Eps/k), then for each form of the neighbourhood
  * kernels (cell-parallel; point-parallel with 1, 2 or 4 lanes per query) at
- * the best reach (each form timed twice, its faster trial kept), then for the
+ * the best reach (each reach and each form timed twice in interleaved
+ * rounds, its faster trial kept), then for the
  * workgroups per CU of the cell-parallel kernels
  * and then of the stencil-rows kernel, and returns the fastest setting.
  * `points` holds W + (slides+1)*S points (slides >= 1: one warm-up slide, then

@@ -22,7 +22,7 @@ constexpr int kMaxSlotBits = 24;  // slot bits of the packed key; the cell key t
 constexpr int kUGCells = 8;  // query cells per wave of the cell-level merge
 constexpr int kMaxBatch = 1024;
 constexpr int kMaxPerCu = 1024;  // largest workgroups-per-CU setting of a persistent grid
-constexpr int kCandPad = 4;      // readable floats past the end of each window coordinate array
+constexpr int kCandPad = 4;      // floats of margin past the end of each window coordinate array
 // Per-stream work counters (optional; rtdbs_stats in the C ABI)
 constexpr int kNumStats = 8;
 enum Stat {

@@ -138,8 +138,8 @@ static Layout make_layout(const rtdbs_params &p, int R) {
     L.off_mut = take(sizeof(StreamMut));
     L.off_skey = take(W * 8);
     // window coordinates (both buffers of the ping-pong) with kCandPad
-    // readable floats past the end: the point-parallel kernels load candidates
-    // in groups that may run past a stencil row's end
+    // readable floats past the end, a margin: the candidate walks read real
+    // candidates only
     const size_t wc = (W + kCandPad) * 4;
     L.off_sx = take(wc);
     L.off_sy = d > 1 ? take(wc) : 0;

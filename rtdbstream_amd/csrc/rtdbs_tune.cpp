@@ -165,20 +165,29 @@ extern "C" int rtdbs_autotune(const rtdbs_params *params, const float *points, i
         out->from_cache = 1;
         return RTDBS_OK;
     }
-    // 1. stencil reach, with the launch-size defaults for the rest
+    // 1. stencil reach, with the launch-size defaults for the rest; like the
+    // forms below, each reach timed twice in interleaved rounds, its faster
+    // trial kept
     Choice best;
+    double reach_ms[kMaxK + 1];
+    for (int kk = 1; kk <= kMaxK; kk++) reach_ms[kk] = 1e300;
+    for (int round = 0; round < 2; round++) {
+        for (int kk = 1; kk <= kMaxK; kk++) {
+            rtdbs_params p = p0;
+            p.cell_div = kk;
+            p.lanes_per_query = 0;
+            p.neighbor_mode = 0;
+            double m = 0.0;
+            rc = trial(p, points, slides, &m);
+            if (rc) return rc;
+            out->trials++;
+            reach_ms[kk] = std::min(reach_ms[kk], m);
+        }
+    }
     best.ms = 1e300;
     for (int kk = 1; kk <= kMaxK; kk++) {
-        rtdbs_params p = p0;
-        p.cell_div = kk;
-        p.lanes_per_query = 0;
-        p.neighbor_mode = 0;
-        double m = 0.0;
-        rc = trial(p, points, slides, &m);
-        if (rc) return rc;
-        out->trials++;
-        if (m < best.ms) {
-            best.ms = m;
+        if (reach_ms[kk] < best.ms) {
+            best.ms = reach_ms[kk];
             best.k = kk;
         }
     }
