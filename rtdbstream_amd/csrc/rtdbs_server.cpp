@@ -154,9 +154,11 @@ bool swap_in_one(rtdbs_server *srv, std::unique_lock<std::mutex> &lk) {
     srv->bytes_in_use = srv->bytes_in_use - freed + ss->bytes;
     lk.unlock();
     size_t kept = 0;  // victims whose checkpoint failed stay on the device
+    std::string why;  // the first failed checkpoint's message
     for (SStream *v : victims) {
         std::vector<float> win((size_t)v->s->p.window * v->s->p.dim);
         const int rc = rtdbs_stream_get_window(v->s, win.data());
+        if (rc != RTDBS_OK && why.empty()) why = rtdbs_last_error();
         const int64_t slides = v->s->slides;
         if (rc == RTDBS_OK) rtdbs_stream_destroy(v->s);
         std::lock_guard<std::mutex> g(srv->mu);
@@ -171,7 +173,7 @@ bool swap_in_one(rtdbs_server *srv, std::unique_lock<std::mutex> &lk) {
         v->busy = false;
     }
     rtdbs_stream *st = nullptr;
-    int rc = kept ? fail(RTDBS_ENOMEM, "a stream could not be checkpointed to make room") : RTDBS_OK;
+    int rc = kept ? fail(RTDBS_ENOMEM, "a stream could not be checkpointed to make room (%s)", why.c_str()) : RTDBS_OK;
     if (!rc) rc = rtdbs_stream_create(&ss->params, &st);
     if (!rc) rc = rtdbs_stream_restore(st, ss->ckpt.data(), ss->ckpt_slides);
     lk.lock();
